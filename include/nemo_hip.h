@@ -1,0 +1,135 @@
+#ifndef NEMO_HIP_H
+#define NEMO_HIP_H
+
+/*
+ * nemo_hip.h -- extensions of the NeMo C API specific to the HIP backend.
+ *
+ * NeMo's API constructs networks one synapse per call and steps one cycle
+ * per call with a host round trip. The functions here add the forms that a
+ * GPU needs to be fed at full rate: bulk construction from arrays, runs of
+ * many cycles that stay on the device, and batched runs of many independent
+ * networks in one persistent launch. None of them changes the model
+ * arithmetic: a run of K cycles gives the same rasters, state and weights as
+ * K calls of nemo_step().
+ */
+
+#include <stddef.h>
+#include <stdint.h>
+#include "nemo.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ construction */
+
+/* Bulk form of nemo_add_neuron (src/nemo.h:265-268): neuron i has index
+ * idx[i] and arguments args[i*nargs .. i*nargs+nargs). */
+nemo_status_t nemo_add_neurons(nemo_network_t net, unsigned type, size_t n,
+		const unsigned idx[], unsigned nargs, const float args[]);
+
+/* Bulk form of nemo_add_synapse (src/nemo.h:278-286) for whole axons:
+ * source k (sources[k], or k when sources is NULL) gets synapses
+ * rowptr[k] .. rowptr[k+1]-1 of the arrays, in that order, so its synapse
+ * ids are (source << 32) | (previous count + position). plastic may be NULL
+ * (all static). */
+nemo_status_t nemo_add_synapses_csr(nemo_network_t net, size_t nsources,
+		const unsigned sources[], const uint64_t rowptr[],
+		const unsigned targets[], const unsigned char delays[],
+		const float weights[], const unsigned char plastic[]);
+
+/* ----------------------------------------------------------- configuration */
+
+/* Same as nemo_set_cuda_backend: run on HIP device dev (dev < 0: device 0). */
+nemo_status_t nemo_set_hip_backend(nemo_configuration_t conf, int dev);
+
+/* Neurons per partition (workgroup) of the step kernel, rounded up to a
+ * multiple of 64; 0 = automatic. Counterpart of NeMo's
+ * Configuration::setCudaPartitionSize (src/nemo/Configuration.hpp:61). */
+nemo_status_t nemo_set_partition_size(nemo_configuration_t conf, unsigned neurons);
+
+/* ------------------------------------------------------- multi-cycle runs */
+
+/*
+ * Stimulus of a run, as CSR over the run's cycles: cycle k forces the
+ * neurons fstim_idx[fstim_ptr[k] .. fstim_ptr[k+1]) to fire and assigns
+ * istim_val[j] to the external current of neuron istim_idx[j] for
+ * j in istim_ptr[k] .. istim_ptr[k+1] (the last assignment of a neuron in a
+ * cycle wins, as nemo_step's addCurrentStimulus). Indices are user neuron
+ * indices. Any of the pointer pairs may be NULL.
+ */
+typedef struct nemo_schedule {
+	const unsigned* fstim_ptr;
+	const unsigned* fstim_idx;
+	const unsigned* istim_ptr;
+	const unsigned* istim_idx;
+	const float* istim_val;
+} nemo_schedule_t;
+
+/* Run nsteps cycles on the device. The firing raster of the run stays on
+ * the device; read it with nemo_hip_read_fired / nemo_hip_read_raster.
+ * events (may be NULL) receives the number of synapses delivered. */
+nemo_status_t nemo_hip_run(nemo_simulation_t sim, unsigned nsteps,
+		const nemo_schedule_t* schedule, unsigned long long* events);
+
+/* Fired neurons of the last run: counts[k] for each of its cycles, and the
+ * user indices of all fired neurons back to back (cycle order, within a
+ * cycle in the order nemo_step reports them). ids may be NULL; cap is its
+ * capacity; *total receives the number of ids. */
+nemo_status_t nemo_hip_read_fired(nemo_simulation_t sim, unsigned counts[],
+		unsigned ids[], size_t cap, size_t* total);
+
+/* Raw raster of the last run: words_per_step 64-bit words per cycle; bit b
+ * of word w is simulation-local neuron 64*w+b. */
+nemo_status_t nemo_hip_read_raster(nemo_simulation_t sim, uint64_t words[], size_t cap,
+		size_t* words_per_step);
+
+/* ------------------------------------------------------------------ batches */
+
+/*
+ * Run nsteps[i] cycles of each of n independent simulations in persistent
+ * launches that hold as many of them on the device at once as fit (one
+ * launch if all fit). Each simulation keeps its own state, RNG streams,
+ * stimulus-free dynamics and raster, identical to running it alone;
+ * events[i] (may be NULL) receives each one's delivered synapses.
+ */
+nemo_status_t nemo_hip_run_batch(nemo_simulation_t sims[], size_t n,
+		const unsigned nsteps[], unsigned long long events[]);
+
+/* ------------------------------------------------------------------- info */
+
+typedef struct nemo_hip_info {
+	unsigned neurons;
+	unsigned partitions;         /* workgroups of the step kernel */
+	unsigned partition_size;     /* neurons per partition */
+	unsigned max_delay;
+	unsigned saturating;         /* 1 if accumulators may leave int32 */
+	unsigned stdp;
+	unsigned long long synapses;
+	unsigned long long plastic_synapses;
+	unsigned long long device_bytes;
+	unsigned lds_bytes;
+	unsigned block_threads;
+	float last_kernel_ms;        /* step-kernel time of the last run (HIP events) */
+} nemo_hip_info_t;
+
+nemo_status_t nemo_hip_get_info(nemo_simulation_t sim, nemo_hip_info_t* info);
+
+/* Status of the most recent API call (nemo_new_simulation returns NULL on
+ * failure; this gives its status code). */
+nemo_status_t nemo_hip_last_status(void);
+
+/* Evaluate the device's logf / sinf of the RNG (kind 0 / 1) on n host
+ * floats, for checking them against the host C library. With on_device = 0
+ * the same code runs on the host CPU (no GPU needed). */
+nemo_status_t nemo_hip_eval_libm(int kind, const float in[], float out[], size_t n, int on_device);
+
+/* The Gaussian sequence nrand() yields from RNG state state4, computed on
+ * the device. */
+nemo_status_t nemo_hip_nrand_seq(const uint32_t state4[4], float out[], size_t n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,150 @@
+/*
+ * libm_f32.h -- single-precision logf / sinf that reproduce the host C
+ * library bit for bit on the inputs of NeMo's Gaussian RNG.
+ *
+ * NeMo's nrand() (src/nemo/RNG.cpp:25-33) calls the host's logf and sinf.
+ * glibc >= 2.28 implements both with the algorithms published in ARM's
+ * "optimized-routines" project (math/logf.c, math/sinf.c, math/sincosf.h,
+ * math/logf_data.c, math/sincosf_data.c): a 16-entry table plus a degree-3
+ * polynomial in double for logf, and a quadrant reduction plus even/odd
+ * double polynomials for sinf. This file restates those algorithms in
+ * double precision so they run on gfx950 FP64 and on the host alike.
+ *
+ * The results are the same whether or not the compiler fuses the
+ * multiply-adds (glibc selects an FMA build on FMA-capable x86 hosts):
+ * both forms were compared with the host libm on every input the RNG can
+ * produce -- logf at all x = m * 2^-24 (m = 0..2^24) and sinf at all
+ * a = float(u) * 1.4629180792671596810513378043098e-9f, u a 32-bit unsigned
+ * -- with zero differences. tests/test_libm_f32.py repeats that check.
+ *
+ * Only the domains above are supported: logf for x in [0, 1], sinf for
+ * a in [0, 2*pi + 1ulp]. Other inputs are not handled (no NaN / negative /
+ * large-argument paths).
+ */
+#ifndef NEMOSIM_LIBM_F32_H
+#define NEMOSIM_LIBM_F32_H
+
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#define NEMO_HD __host__ __device__
+#else
+#define NEMO_HD
+#endif
+
+namespace nemo_libm {
+
+NEMO_HD inline uint32_t f2u(float f) { union { float f; uint32_t u; } x; x.f = f; return x.u; }
+NEMO_HD inline float u2f(uint32_t u) { union { float f; uint32_t u; } x; x.u = u; return x.f; }
+
+/* 1/c and log(c) for the 16 sub-intervals of [0x3f330000, 2*0x3f330000) */
+struct LogfEntry { double invc, logc; };
+
+/* constexpr, so that clang makes the table available to device code too */
+constexpr LogfEntry LOGF_TABLE[16] = {
+	{ 0x1.661ec79f8f3bep+0, -0x1.57bf7808caadep-2 },
+	{ 0x1.571ed4aaf883dp+0, -0x1.2bef0a7c06ddbp-2 },
+	{ 0x1.49539f0f010bp+0, -0x1.01eae7f513a67p-2 },
+	{ 0x1.3c995b0b80385p+0, -0x1.b31d8a68224e9p-3 },
+	{ 0x1.30d190c8864a5p+0, -0x1.6574f0ac07758p-3 },
+	{ 0x1.25e227b0b8eap+0, -0x1.1aa2bc79c81p-3 },
+	{ 0x1.1bb4a4a1a343fp+0, -0x1.a4e76ce8c0e5ep-4 },
+	{ 0x1.12358f08ae5bap+0, -0x1.1973c5a611cccp-4 },
+	{ 0x1.0953f419900a7p+0, -0x1.252f438e10c1ep-5 },
+	{ 0x1p+0, 0x0p+0 },
+	{ 0x1.e608cfd9a47acp-1, 0x1.aa5aa5df25984p-5 },
+	{ 0x1.ca4b31f026aap-1, 0x1.c5e53aa362eb4p-4 },
+	{ 0x1.b2036576afce6p-1, 0x1.526e57720db08p-3 },
+	{ 0x1.9c2d163a1aa2dp-1, 0x1.bc2860d22477p-3 },
+	{ 0x1.886e6037841edp-1, 0x1.1058bc8a07ee1p-2 },
+	{ 0x1.767dcf5534862p-1, 0x1.4043057b6ee09p-2 },
+};
+
+/* logf for x in [0, 1]: x = 2^k z with z in [OFF, 2 OFF); log(x) =
+ * log1p(z/c - 1) + log(c) + k ln2, log1p by a degree-3 polynomial. */
+NEMO_HD inline float logf_glibc(float x)
+{
+	const double A0 = -0x1.00ea348b88334p-2;
+	const double A1 = 0x1.5575b0be00b6ap-2;
+	const double A2 = -0x1.ffffef20a4123p-2;
+	const double LN2 = 0x1.62e42fefa39efp-1;
+	const uint32_t OFF = 0x3f330000u;
+
+	uint32_t ix = f2u(x);
+	if(ix == 0x3f800000u) {
+		return 0.0f;
+	}
+	if(ix == 0u) {
+		return -__builtin_inff();
+	}
+	uint32_t tmp = ix - OFF;
+	int i = int((tmp >> 19) & 15u);
+	int k = int32_t(tmp) >> 23;
+	uint32_t iz = ix - (tmp & 0xff800000u);
+	const LogfEntry& e = LOGF_TABLE[i];
+	double z = double(u2f(iz));
+	double r = z * e.invc - 1.0;
+	double y0 = e.logc + double(k) * LN2;
+	double r2 = r * r;
+	double y = A1 * r + A2;
+	y = A0 * r2 + y;
+	y = y * r2 + (y0 + r);
+	return float(y);
+}
+
+/* Sine / cosine polynomial set; the second set negates the cosine
+ * coefficients for quadrants 2 and 3. */
+struct SincosfSet {
+	double c0, c1, c2, c3, c4, s1, s2, s3;
+};
+
+NEMO_HD inline float sinf_poly(double x, double x2, const SincosfSet& p, int n)
+{
+	if((n & 1) == 0) {
+		double x3 = x * x2;
+		double s1 = p.s2 + x2 * p.s3;
+		double x7 = x3 * x2;
+		double s = x + x3 * p.s1;
+		return float(s + x7 * s1);
+	} else {
+		double x4 = x2 * x2;
+		double c2 = p.c3 + x2 * p.c4;
+		double c1 = p.c0 + x2 * p.c1;
+		double x6 = x4 * x2;
+		double c = c1 + x4 * p.c2;
+		return float(c + x6 * c2);
+	}
+}
+
+/* sinf for a in [0, 2*pi]: |a| < pi/4 directly, otherwise one-step
+ * reduction by pi/2 (rounded through a 2^24-scaled integer). */
+NEMO_HD inline float sinf_glibc(float y)
+{
+	const SincosfSet P0 = { 0x1p0, -0x1.ffffffd0c621cp-2, 0x1.55553e1068f19p-5,
+		-0x1.6c087e89a359dp-10, 0x1.99343027bf8c3p-16,
+		-0x1.555545995a603p-3, 0x1.1107605230bc4p-7, -0x1.994eb3774cf24p-13 };
+	const SincosfSet P1 = { -0x1p0, 0x1.ffffffd0c621cp-2, -0x1.55553e1068f19p-5,
+		0x1.6c087e89a359dp-10, -0x1.99343027bf8c3p-16,
+		-0x1.555545995a603p-3, 0x1.1107605230bc4p-7, -0x1.994eb3774cf24p-13 };
+	const double HPI_INV = 0x1.45F306DC9C883p+23;
+	const double HPI = 0x1.921FB54442D18p0;
+
+	uint32_t top = (f2u(y) >> 20) & 0x7ffu;
+	double x = y;
+	if(top < ((f2u(0x1.921FB6p-1f) >> 20) & 0x7ffu)) {
+		if(top < ((f2u(0x1p-12f) >> 20) & 0x7ffu)) {
+			return y;
+		}
+		return sinf_poly(x, x * x, P0, 0);
+	}
+	double r = x * HPI_INV;
+	int n = (int32_t(r) + 0x800000) >> 24;
+	x = x - double(n) * HPI;
+	double s = ((n & 3) == 1 || (n & 3) == 2) ? -1.0 : 1.0;
+	const SincosfSet& p = (n & 2) ? P1 : P0;
+	return sinf_poly(x * s, x * x, p, n);
+}
+
+} // namespace nemo_libm
+
+#endif

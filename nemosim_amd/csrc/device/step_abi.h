@@ -1,0 +1,117 @@
+/*
+ * step_abi.h -- plain-old-data structures shared by the C++ host code and
+ * the HIP kernels (the internal C-ABI between them). No HIP types appear
+ * here, so the host side compiles against it with any C++ compiler.
+ */
+#ifndef NEMOSIM_STEP_ABI_H
+#define NEMOSIM_STEP_ABI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Bits of NetDev.flags */
+#define NEMO_NET_STDP      1u   /* accumulate STDP this launch */
+#define NEMO_NET_SATURATE  2u   /* accumulators may leave the int32 range */
+
+/* Bits of the per-network status word */
+#define NEMO_STATUS_BARRIER_TIMEOUT 1u
+
+/*
+ * One simulated network as seen by the step kernel. Neurons use
+ * simulation-local indices 0..n-1; partition p owns neurons [p*T, p*T+T).
+ * All arrays are device pointers.
+ */
+typedef struct NetDev {
+	uint32_t n;          /* neurons */
+	uint32_t P;          /* partitions (workgroups) */
+	uint32_t T;          /* neurons per partition, multiple of 64, <= 1024 */
+	uint32_t D;          /* delay-ring depth = maximum delay (1..64) */
+	uint32_t words;      /* 64-bit raster words per step = P*T/64 */
+	uint32_t flags;      /* NEMO_NET_* */
+	uint32_t nsteps;     /* steps this network runs in the current launch */
+	uint32_t spinLimit;  /* barrier spin bound (iterations) */
+	uint64_t cycle0;     /* cycle count at the start of the launch */
+
+	const uint8_t* model;     /* [n] model id */
+	float* param;             /* [MAX_PARAM][n] */
+	float* state;             /* [MAX_STATE][n] */
+	uint32_t* rng;            /* [4][n] */
+	uint64_t* recent;         /* [n] firing history, bit 0 = last cycle */
+	int32_t* ring;            /* [P][2][D][T] pending current (saved between launches) */
+
+	/* forward matrix: synapses of source s bucketed by target partition;
+	 * partition p's bucket is syn[off[s*(P+1)+p] .. off[s*(P+1)+p+1]).
+	 * Record = {target - p*T | delay << 16, Q11.20 weight}. */
+	const uint32_t* off;
+	uint32_t* syn;            /* [2 * synapses] */
+
+	/* per-step exchange between the network's workgroups */
+	uint32_t* fl;             /* [2][P][T] fired neurons of each partition */
+	uint32_t* fc;             /* [2][P] their counts */
+	uint64_t* rfg;            /* [2][n] firing history (STDP only) */
+	uint32_t* bar;            /* barrier arrival counter, zeroed before a launch */
+	uint32_t* status;         /* NEMO_STATUS_* */
+	unsigned long long* events; /* delivered-synapse counter */
+
+	/* reverse matrix of plastic synapses (STDP) */
+	const uint32_t* rcm_start;   /* [n+1] */
+	const uint32_t* rcm_sd;      /* source | (delay-1) << 26 */
+	const uint32_t* rcm_fwd;     /* index of the synapse record */
+	int32_t* rcm_acc;            /* accumulated weight change */
+	const void* stdp;            /* StdpParams */
+
+	/* stimulus of this launch, CSR over steps (sim-local indices); NULL if none */
+	const uint32_t* fs_ptr;
+	const uint32_t* fs_idx;
+	const uint32_t* is_ptr;
+	const uint32_t* is_idx;
+	const float* is_val;
+
+	uint64_t* raster;         /* [nsteps][words] fired bits */
+} NetDev;
+
+/* blockIdx -> (network, partition) */
+typedef struct WorkItem {
+	uint32_t net;
+	uint32_t part;
+} WorkItem;
+
+/* Kernel launchers (nemosim_amd/csrc/device/step_kernel.hip). They return a
+ * hipError_t value as int and take a hipStream_t as void*. */
+int nemo_dev_step_launch(const NetDev* nets_dev, const WorkItem* work_dev, uint32_t nblocks,
+		uint32_t blockDim, uint32_t ldsBytes, void* stream);
+int nemo_dev_step_max_blocks(uint32_t blockDim, uint32_t ldsBytes, uint32_t* maxBlocks);
+int nemo_dev_apply_stdp(uint32_t* syn, const uint32_t* rcm_fwd, int32_t* rcm_acc, size_t nr,
+		const void* stdp_dev, int32_t fx_reward, void* stream);
+int nemo_dev_fill_u32(uint32_t* p, uint32_t value, size_t n, void* stream);
+int nemo_dev_fill_u64(uint64_t* p, uint64_t value, size_t n, void* stream);
+/* Evaluates the device nrand arithmetic pieces on inputs in[] (for tests):
+ * kind 0 = logf_glibc(in), 1 = sinf_glibc(in), 2 = nrand sequence from state in[0..3] */
+int nemo_dev_eval_libm(int kind, const float* in_dev, float* out_dev, size_t n, void* stream);
+int nemo_dev_nrand_seq(const uint32_t* state4_dev, float* out_dev, size_t n, void* stream);
+
+/* LDS bytes the step kernel needs for partitions of T neurons, ring depth D,
+ * P partitions. */
+static inline uint32_t nemo_step_lds_bytes(uint32_t T, uint32_t D, uint32_t P, int saturate)
+{
+	uint32_t b = 0;
+	b += 2u * D * T * 4u;          /* ringE, ringI */
+	b += T * 4u;                   /* external current / STDP target list */
+	b += (T / 32u) * 4u;           /* forced-firing bits */
+	b += ((P + 1u + 63u) & ~63u) * 4u; /* partition prefix */
+	b += 64u * 4u;                 /* wave sums / scratch */
+	if(saturate) {
+		b += 2u * D * (T / 32u) * 4u;  /* overflow bits */
+	}
+	return b;
+}
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

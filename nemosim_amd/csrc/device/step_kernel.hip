@@ -1,0 +1,508 @@
+/*
+ * step_kernel.hip -- the simulation step on gfx950.
+ *
+ * One persistent, cooperatively launched kernel runs many 1 ms cycles of one
+ * or more independent networks. Each workgroup owns one partition of T
+ * target neurons of one network:
+ *
+ *   - the partition's neuron parameters, state, RNG and 64-cycle firing
+ *     history stay in registers for the whole launch (one neuron per thread);
+ *   - incoming synaptic current for the next D cycles is accumulated in an
+ *     LDS ring [D][T] of Q11.20 int32 (separate excitatory and inhibitory
+ *     planes, as NeMo's CPU backend sums them, src/nemo/cpu/Simulation.cpp:
+ *     292-307); LDS integer atomics make the sum order-independent, so the
+ *     result equals the CPU's int64 sum whenever no partial sum leaves the
+ *     int32 range -- networks where one might are run with returning atomics
+ *     and sticky overflow bits (src/nemo/cuda/fixedpoint.cu:69-81 does the
+ *     same), which saturates exactly like wfx_toFloat.
+ *
+ * Cycle t of partition p (the CPU's fire(), src/nemo/cpu/Simulation.cpp:94-111):
+ *   1. current_t = ring[t mod D]; neuron update (Izhikevich / IF_curr_exp,
+ *      per-neuron xorshift + Box-Muller); firing history shift;
+ *   2. fired-list compaction: wavefront ballot + prefix over waves, giving
+ *      the partition's fired neurons in ascending order, and the 64-bit
+ *      ballot words as the step's firing raster;
+ *   3. barrier of the network's workgroups (the only grid-level sync);
+ *   4. delivery: every workgroup walks the fired lists of all partitions and,
+ *      for each fired source, reads only its own bucket of the source's
+ *      synapses (all delays at once) and adds each weight into ring slot
+ *      (t + delay) mod D -- the delay-bucketed spike queue lives in LDS;
+ *   5. STDP: targets that fired postLen cycles ago walk their plastic
+ *      in-synapses and accumulate weight changes
+ *      (src/nemo/ConnectivityMatrix.cpp:193-227).
+ *
+ * Blocks are assigned to partitions by the host so that each XCD (blocks
+ * b, b+8, ...) gets a contiguous range of partitions: the synapse buckets of
+ * one source are then contiguous per XCD and each XCD's L2 fetches only its
+ * slice of every row.
+ */
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "models.h"
+#include "step_abi.h"
+
+using namespace nemo_dev;
+
+namespace {
+
+__device__ inline uint32_t lds_prefix_waves(uint32_t* wsum, uint32_t wave, uint32_t nwaves, uint32_t* total)
+{
+	uint32_t before = 0, all = 0;
+	for(uint32_t w = 0; w < nwaves; ++w) {
+		uint32_t c = wsum[w];
+		before += (w < wave) ? c : 0u;
+		all += c;
+	}
+	*total = all;
+	return before;
+}
+
+/* Barrier of the workgroups of one network. bar counts arrivals since the
+ * launch started; step k waits for (k+1)*P of them. The wait is bounded:
+ * after spinLimit polls the status word is flagged and every later barrier of
+ * the launch falls through, so all waves always run to completion. */
+__device__ inline void net_barrier(uint32_t* bar, uint32_t target, uint32_t spinLimit, uint32_t* status)
+{
+	__syncthreads();
+	if(threadIdx.x == 0) {
+		__threadfence();
+		atomicAdd(bar, 1u);
+		if(__hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
+			uint32_t spins = 0;
+			while(__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+				__builtin_amdgcn_s_sleep(1);
+				if(++spins > spinLimit) {
+					atomicOr(status, NEMO_STATUS_BARRIER_TIMEOUT);
+					break;
+				}
+			}
+		}
+		__threadfence();
+	}
+	__syncthreads();
+}
+
+template<bool SAT>
+__device__ void run_partition(const NetDev& nd, uint32_t p)
+{
+	extern __shared__ __align__(16) uint8_t lds[];
+
+	const uint32_t T = nd.T;
+	const uint32_t D = nd.D;
+	const uint32_t P = nd.P;
+	const uint32_t n_total = nd.n;
+	const uint32_t tid = threadIdx.x;
+	const uint32_t nthreads = blockDim.x;
+	const uint32_t wave = tid >> 6;
+	const uint32_t lane = tid & 63u;
+	const uint32_t nwaves = nthreads >> 6;
+	const uint32_t n = p * T + tid;
+	const bool active = (tid < T) && (n < n_total);
+
+	int32_t* ringE = reinterpret_cast<int32_t*>(lds);
+	int32_t* ringI = ringE + D * T;
+	float* ext = reinterpret_cast<float*>(ringI + D * T);
+	uint32_t* qlist = reinterpret_cast<uint32_t*>(ext);      /* aliases ext (STDP phase only) */
+	uint32_t* fst = reinterpret_cast<uint32_t*>(ext + T);
+	uint32_t* pref = fst + (T >> 5);
+	uint32_t* wsum = pref + ((P + 1u + 63u) & ~63u);
+	uint32_t* ovf = wsum + 64;                                 /* [2][D][T/32] when SAT */
+	const uint32_t fwords = T >> 5;
+
+	/* restore the pending-current ring */
+	const int32_t* gring = nd.ring + size_t(p) * 2u * D * T;
+	for(uint32_t i = tid; i < 2u * D * T; i += nthreads) {
+		ringE[i] = gring[i];
+	}
+	if(SAT) {
+		for(uint32_t i = tid; i < 2u * D * fwords; i += nthreads) {
+			ovf[i] = 0u;
+		}
+	}
+
+	/* neuron registers */
+	int model = 0;
+	float prm[MAX_PARAM];
+	float st[MAX_STATE];
+	Rng rng;
+	uint64_t recent = 0;
+#pragma unroll
+	for(int k = 0; k < MAX_PARAM; ++k) prm[k] = 0.0f;
+#pragma unroll
+	for(int k = 0; k < MAX_STATE; ++k) st[k] = 0.0f;
+	rng.s[0] = rng.s[1] = rng.s[2] = rng.s[3] = 0u;
+	if(active) {
+		model = nd.model[n];
+#pragma unroll
+		for(int k = 0; k < MAX_PARAM; ++k) prm[k] = nd.param[size_t(k) * n_total + n];
+#pragma unroll
+		for(int k = 0; k < MAX_STATE; ++k) st[k] = nd.state[size_t(k) * n_total + n];
+#pragma unroll
+		for(int k = 0; k < 4; ++k) rng.s[k] = nd.rng[size_t(k) * n_total + n];
+		recent = nd.recent[n];
+	}
+
+	const bool stdp = (nd.flags & NEMO_NET_STDP) != 0u;
+	const StdpParams* sp = reinterpret_cast<const StdpParams*>(nd.stdp);
+	const uint32_t postLen = stdp ? uint32_t(sp->postLen) : 0u;
+	unsigned long long events = 0;
+	__syncthreads();
+
+	for(uint32_t k = 0; k < nd.nsteps; ++k) {
+		const uint64_t t = nd.cycle0 + k;
+		const uint32_t slot = uint32_t(t % D);
+		const uint32_t par = uint32_t(t & 1u);
+
+		/* ---- stimulus of this step (sim-local indices, deduplicated by the host) */
+		bool has_stim = false;
+		uint32_t fs0 = 0, fs1 = 0, is0 = 0, is1 = 0;
+		if(nd.fs_ptr) { fs0 = nd.fs_ptr[k]; fs1 = nd.fs_ptr[k + 1]; }
+		if(nd.is_ptr) { is0 = nd.is_ptr[k]; is1 = nd.is_ptr[k + 1]; }
+		has_stim = (fs1 > fs0) || (is1 > is0);
+		if(has_stim) {
+			if(tid < T) ext[tid] = 0.0f;
+			if(tid < fwords) fst[tid] = 0u;
+			__syncthreads();
+			const uint32_t lo = p * T, hi = p * T + T;
+			for(uint32_t e = fs0 + tid; e < fs1; e += nthreads) {
+				uint32_t idx = nd.fs_idx[e];
+				if(idx >= lo && idx < hi) {
+					uint32_t l = idx - lo;
+					atomicOr(&fst[l >> 5], 1u << (l & 31u));
+				}
+			}
+			for(uint32_t e = is0 + tid; e < is1; e += nthreads) {
+				uint32_t idx = nd.is_idx[e];
+				if(idx >= lo && idx < hi) {
+					ext[idx - lo] = nd.is_val[e];
+				}
+			}
+			__syncthreads();
+		}
+
+		/* ---- 1. neuron update */
+		bool fired = false;
+		if(active) {
+			const uint32_t r = slot * T + tid;
+			int32_t e = ringE[r];
+			int32_t i = ringI[r];
+			if(SAT) {
+				const uint32_t bit = 1u << (tid & 31u);
+				if(ovf[slot * fwords + (tid >> 5)] & bit) e = INT32_MAX;
+				if(ovf[(D + slot) * fwords + (tid >> 5)] & bit) i = INT32_MIN;
+			}
+			ringE[r] = 0;
+			ringI[r] = 0;
+			float cx = 0.0f;
+			bool fs = false;
+			if(has_stim) {
+				cx = ext[tid];
+				fs = ((fst[tid >> 5] >> (tid & 31u)) & 1u) != 0u;
+			}
+			fired = update_neuron(model, prm, st, rng, fx20_to_float(e), fx20_to_float(i), cx, fs);
+			recent = (recent << 1) | (fired ? 1ull : 0ull);
+			if(stdp) {
+				nd.rfg[size_t(par) * n_total + n] = recent;
+			}
+		}
+
+		/* ---- 2. compaction: ballot + prefix over waves */
+		const uint64_t mask = __ballot(fired);
+		if(lane == 0) {
+			wsum[wave] = uint32_t(__popcll(mask));
+			if(wave < (T >> 6)) {
+				nd.raster[size_t(k) * nd.words + p * (T >> 6) + wave] = mask;
+			}
+		}
+		__syncthreads();
+		if(SAT) {
+			/* the slot just consumed starts empty for cycle t + D */
+			for(uint32_t w = tid; w < fwords; w += nthreads) {
+				ovf[slot * fwords + w] = 0u;
+				ovf[(D + slot) * fwords + w] = 0u;
+			}
+		}
+		{
+			uint32_t total;
+			uint32_t base = lds_prefix_waves(wsum, wave, nwaves, &total);
+			if(fired) {
+				const uint64_t below = lane ? (mask & (~0ull >> (64u - lane))) : 0ull;
+				const uint32_t pos = base + uint32_t(__popcll(below));
+				nd.fl[(size_t(par) * P + p) * T + pos] = n;
+			}
+			if(tid == 0) {
+				nd.fc[par * P + p] = total;
+			}
+		}
+
+		/* ---- 3. the network's barrier */
+		net_barrier(nd.bar, (k + 1u) * P, nd.spinLimit, nd.status);
+
+		/* ---- 4. delivery of this cycle's spikes into the ring */
+		if(wave == 0) {
+			uint32_t carry = 0;
+			for(uint32_t c = 0; c < P; c += 64u) {
+				uint32_t v = (c + lane < P) ? nd.fc[par * P + c + lane] : 0u;
+				uint32_t x = v;
+				/* inclusive wave scan */
+#pragma unroll
+				for(uint32_t d = 1; d < 64u; d <<= 1) {
+					uint32_t y = __shfl_up(x, d, 64);
+					if(lane >= d) x += y;
+				}
+				if(c + lane < P) pref[c + lane + 1] = carry + x;
+				carry += __shfl(x, 63, 64);
+			}
+			if(lane == 0) pref[0] = 0u;
+		}
+		__syncthreads();
+		{
+			const uint32_t F = pref[P];
+			const uint32_t* fl = nd.fl + size_t(par) * P * T;
+			const uint32_t* off = nd.off;
+			const uint2* syn = reinterpret_cast<const uint2*>(nd.syn);
+			for(uint32_t j = tid; j < F; j += nthreads) {
+				uint32_t lo = 0, hi = P;
+				while(hi - lo > 1u) {
+					uint32_t mid = (lo + hi) >> 1;
+					if(pref[mid] <= j) lo = mid; else hi = mid;
+				}
+				const uint32_t s = fl[size_t(lo) * T + (j - pref[lo])];
+				const uint32_t* o = off + size_t(s) * (P + 1u) + p;
+				const uint32_t a = o[0];
+				const uint32_t b = o[1];
+				events += b - a;
+				for(uint32_t q = a; q < b; ++q) {
+					const uint2 rec = syn[q];
+					const uint32_t tl = rec.x & 0xffffu;
+					uint32_t sl = slot + (rec.x >> 16);
+					sl = (sl >= D) ? sl - D : sl;
+					const int32_t w = int32_t(rec.y);
+					int32_t* acc = (w >= 0) ? ringE : ringI;
+					if(SAT) {
+						int32_t old = __hip_atomic_fetch_add(&acc[sl * T + tl], w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+						int64_t sum = int64_t(old) + int64_t(w);
+						if(sum > int64_t(INT32_MAX) || sum < int64_t(INT32_MIN)) {
+							uint32_t plane = (w >= 0) ? 0u : D;
+							atomicOr(&ovf[(plane + sl) * fwords + (tl >> 5)], 1u << (tl & 31u));
+						}
+					} else {
+						__hip_atomic_fetch_add(&acc[sl * T + tl], w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+					}
+				}
+			}
+		}
+
+		/* ---- 5. STDP accumulation for targets that fired postLen cycles ago */
+		if(stdp) {
+			const bool q = active && (((recent >> postLen) & 1ull) != 0ull);
+			const uint64_t qm = __ballot(q);
+			__syncthreads();   /* wsum reuse; also orders delivery before qlist writes */
+			if(lane == 0) wsum[wave] = uint32_t(__popcll(qm));
+			__syncthreads();
+			uint32_t nq;
+			uint32_t base = lds_prefix_waves(wsum, wave, nwaves, &nq);
+			if(q) {
+				const uint64_t below = lane ? (qm & (~0ull >> (64u - lane))) : 0ull;
+				qlist[base + uint32_t(__popcll(below))] = n;
+			}
+			__syncthreads();
+			const uint64_t* rf = nd.rfg + size_t(par) * n_total;
+			for(uint32_t qi = 0; qi < nq; ++qi) {
+				const uint32_t tgt = qlist[qi];
+				const uint32_t r0 = nd.rcm_start[tgt];
+				const uint32_t r1 = nd.rcm_start[tgt + 1];
+				for(uint32_t r = r0 + tid; r < r1; r += nthreads) {
+					const uint32_t sd = nd.rcm_sd[r];
+					const uint32_t src = sd & 0x3ffffffu;
+					const uint32_t d = (sd >> 26) + 1u;
+					const uint64_t pre = rf[src] >> d;
+					const int32_t wd = stdp_weight_change(*sp, pre);
+					if(wd != 0) {
+						nd.rcm_acc[r] = int32_t(uint32_t(nd.rcm_acc[r]) + uint32_t(wd));
+					}
+				}
+			}
+		}
+		__syncthreads();
+	}
+
+	/* ---- save state and the ring for the next launch */
+	if(active) {
+#pragma unroll
+		for(int k = 0; k < MAX_STATE; ++k) nd.state[size_t(k) * n_total + n] = st[k];
+#pragma unroll
+		for(int k = 0; k < 4; ++k) nd.rng[size_t(k) * n_total + n] = rng.s[k];
+		nd.recent[n] = recent;
+	}
+	int32_t* gout = nd.ring + size_t(p) * 2u * D * T;
+	if(SAT) {
+		/* fold sticky overflow into the saved values */
+		for(uint32_t i = tid; i < D * T; i += nthreads) {
+			uint32_t sl = i / T, tl = i % T;
+			uint32_t bit = 1u << (tl & 31u);
+			int32_t e = ringE[i], ii = ringI[i];
+			if(ovf[sl * fwords + (tl >> 5)] & bit) e = INT32_MAX;
+			if(ovf[(D + sl) * fwords + (tl >> 5)] & bit) ii = INT32_MIN;
+			gout[i] = e;
+			gout[D * T + i] = ii;
+		}
+	} else {
+		for(uint32_t i = tid; i < 2u * D * T; i += nthreads) {
+			gout[i] = ringE[i];
+		}
+	}
+	/* delivered-synapse count */
+	for(uint32_t d = 32; d >= 1; d >>= 1) {
+		events += __shfl_down(events, d, 64);
+	}
+	if(lane == 0 && events) {
+		atomicAdd(nd.events, events);
+	}
+}
+
+} // namespace
+
+__global__ void __launch_bounds__(1024)
+nemo_step_kernel(const NetDev* __restrict__ nets, const WorkItem* __restrict__ work)
+{
+	const WorkItem wi = work[blockIdx.x];
+	const NetDev nd = nets[wi.net];
+	if(nd.flags & NEMO_NET_SATURATE) {
+		run_partition<true>(nd, wi.part);
+	} else {
+		run_partition<false>(nd, wi.part);
+	}
+}
+
+/* applyStdp: src/nemo/ConnectivityMatrix.cpp:241-294. With reward 0 every
+ * accumulator contributes fx_mul(0, acc) = 0, the same as the reference's
+ * clear-then-apply. */
+__global__ void nemo_apply_stdp_kernel(uint32_t* syn, const uint32_t* __restrict__ rcm_fwd,
+		int32_t* __restrict__ rcm_acc, size_t nr, const StdpParams* __restrict__ sp, int32_t fx_reward)
+{
+	for(size_t r = size_t(blockIdx.x) * blockDim.x + threadIdx.x; r < nr; r += size_t(gridDim.x) * blockDim.x) {
+		const size_t f = rcm_fwd[r];
+		const int32_t w_old = int32_t(syn[2 * f + 1]);
+		const int32_t w_new = stdp_updated_weight(*sp, w_old, fx_mul20(fx_reward, rcm_acc[r]));
+		if(w_new != w_old) {
+			syn[2 * f + 1] = uint32_t(w_new);
+		}
+		rcm_acc[r] = 0;
+	}
+}
+
+__global__ void nemo_fill_u32_kernel(uint32_t* p, uint32_t v, size_t n)
+{
+	for(size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
+		p[i] = v;
+	}
+}
+
+__global__ void nemo_fill_u64_kernel(uint64_t* p, uint64_t v, size_t n)
+{
+	for(size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
+		p[i] = v;
+	}
+}
+
+__global__ void nemo_eval_libm_kernel(int kind, const float* in, float* out, size_t n)
+{
+	for(size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
+		out[i] = (kind == 0) ? nemo_libm::logf_glibc(in[i]) : nemo_libm::sinf_glibc(in[i]);
+	}
+}
+
+__global__ void nemo_nrand_seq_kernel(const uint32_t* state4, float* out, size_t n)
+{
+	if(blockIdx.x == 0 && threadIdx.x == 0) {
+		Rng r;
+		for(int k = 0; k < 4; ++k) r.s[k] = state4[k];
+		for(size_t i = 0; i < n; ++i) out[i] = nrand(r);
+	}
+}
+
+static unsigned grid_for(size_t n, unsigned block)
+{
+	size_t g = (n + block - 1) / block;
+	if(g < 1) g = 1;
+	if(g > 4096) g = 4096;
+	return unsigned(g);
+}
+
+extern "C" {
+
+int nemo_dev_step_launch(const NetDev* nets_dev, const WorkItem* work_dev, uint32_t nblocks,
+		uint32_t blockDim, uint32_t ldsBytes, void* stream)
+{
+	hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&nemo_step_kernel),
+			hipFuncAttributeMaxDynamicSharedMemorySize, int(ldsBytes));
+	if(e != hipSuccess) return int(e);
+	void* args[] = { const_cast<NetDev**>(&nets_dev), const_cast<WorkItem**>(&work_dev) };
+	e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&nemo_step_kernel),
+			dim3(nblocks), dim3(blockDim), args, ldsBytes, static_cast<hipStream_t>(stream));
+	return int(e);
+}
+
+int nemo_dev_step_max_blocks(uint32_t blockDim, uint32_t ldsBytes, uint32_t* maxBlocks)
+{
+	hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&nemo_step_kernel),
+			hipFuncAttributeMaxDynamicSharedMemorySize, int(ldsBytes));
+	if(e != hipSuccess) return int(e);
+	int perCU = 0;
+	e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU,
+			reinterpret_cast<const void*>(&nemo_step_kernel), int(blockDim), size_t(ldsBytes));
+	if(e != hipSuccess) return int(e);
+	int dev = 0;
+	e = hipGetDevice(&dev);
+	if(e != hipSuccess) return int(e);
+	int cus = 0;
+	e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+	if(e != hipSuccess) return int(e);
+	*maxBlocks = uint32_t(perCU) * uint32_t(cus);
+	return 0;
+}
+
+int nemo_dev_apply_stdp(uint32_t* syn, const uint32_t* rcm_fwd, int32_t* rcm_acc, size_t nr,
+		const void* stdp_dev, int32_t fx_reward, void* stream)
+{
+	if(nr == 0) return 0;
+	hipLaunchKernelGGL(nemo_apply_stdp_kernel, dim3(grid_for(nr, 256)), dim3(256), 0,
+			static_cast<hipStream_t>(stream), syn, rcm_fwd, rcm_acc, nr,
+			static_cast<const StdpParams*>(stdp_dev), fx_reward);
+	return int(hipGetLastError());
+}
+
+int nemo_dev_fill_u32(uint32_t* p, uint32_t value, size_t n, void* stream)
+{
+	if(n == 0) return 0;
+	hipLaunchKernelGGL(nemo_fill_u32_kernel, dim3(grid_for(n, 256)), dim3(256), 0,
+			static_cast<hipStream_t>(stream), p, value, n);
+	return int(hipGetLastError());
+}
+
+int nemo_dev_fill_u64(uint64_t* p, uint64_t value, size_t n, void* stream)
+{
+	if(n == 0) return 0;
+	hipLaunchKernelGGL(nemo_fill_u64_kernel, dim3(grid_for(n, 256)), dim3(256), 0,
+			static_cast<hipStream_t>(stream), p, value, n);
+	return int(hipGetLastError());
+}
+
+int nemo_dev_eval_libm(int kind, const float* in_dev, float* out_dev, size_t n, void* stream)
+{
+	if(n == 0) return 0;
+	hipLaunchKernelGGL(nemo_eval_libm_kernel, dim3(grid_for(n, 256)), dim3(256), 0,
+			static_cast<hipStream_t>(stream), kind, in_dev, out_dev, n);
+	return int(hipGetLastError());
+}
+
+int nemo_dev_nrand_seq(const uint32_t* state4_dev, float* out_dev, size_t n, void* stream)
+{
+	hipLaunchKernelGGL(nemo_nrand_seq_kernel, dim3(1), dim3(64), 0,
+			static_cast<hipStream_t>(stream), state4_dev, out_dev, n);
+	return int(hipGetLastError());
+}
+
+} // extern "C"

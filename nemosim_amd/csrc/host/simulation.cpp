@@ -1,0 +1,835 @@
+#include "simulation.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <random>
+#include <unordered_map>
+
+#include <hip/hip_runtime.h>
+
+namespace nemo {
+namespace hip {
+
+/* LDS available to one workgroup of the step kernel on gfx950 */
+static const uint32_t LDS_LIMIT = 160u * 1024u;
+
+/* ------------------------------------------------------------ configuration */
+
+void
+ConfigurationImpl::setStdpFunction(const std::vector<float>& pre, const std::vector<float>& post,
+		float fminE, float fmaxE, float fminI, float fmaxI)
+{
+	/* src/nemo/StdpFunction.cpp:12-50 */
+	if(fmaxI > 0.0f || fminI > 0.0f) {
+		throw exception(NEMO_INVALID_INPUT, "STDP function should not have positive limits for inhibitory synapses");
+	}
+	if(fmaxE < 0.0f || fminE < 0.0f) {
+		throw exception(NEMO_INVALID_INPUT, "STDP function should not have negative limits for inhibitory synapses");
+	}
+	if(fmaxE < fminE) {
+		throw exception(NEMO_INVALID_INPUT, "STDP should have maximum excitatory weight >= minimum excitatory weight");
+	}
+	if(fmaxI > fminI) {
+		throw exception(NEMO_INVALID_INPUT, "STDP should have abs(maximum inhibitory weight) >= abs(minimum inhibitory weight)");
+	}
+	if(pre.size() + post.size() > 64) {
+		throw exception(NEMO_INVALID_INPUT, "size of STDP window too large");
+	}
+	stdp = true;
+	prefire = pre;
+	postfire = post;
+	minE = fminE; maxE = fmaxE; minI = fminI; maxI = fmaxI;
+}
+
+/* ------------------------------------------------------------------ DevBuf */
+
+DevBuf::~DevBuf()
+{
+	if(ptr) {
+		(void) hipFree(ptr);
+	}
+}
+
+void
+DevBuf::alloc(size_t b)
+{
+	if(ptr) {
+		(void) hipFree(ptr);
+		ptr = nullptr;
+		bytes = 0;
+	}
+	if(b == 0) return;
+	hipCheck(hipMalloc(&ptr, b), "hipMalloc");
+	bytes = b;
+}
+
+void
+DevBuf::ensure(size_t b)
+{
+	if(b > bytes) {
+		alloc(b);
+	}
+}
+
+void
+DevBuf::upload(const void* src, size_t b)
+{
+	alloc(b);
+	if(b) {
+		hipCheck(hipMemcpy(ptr, src, b, hipMemcpyHostToDevice), "hipMemcpy H2D");
+	}
+}
+
+/* -------------------------------------------------------------- simulation */
+
+/* blockIdx b runs on XCD b mod 8 (round-robin dispatch); give each XCD a
+ * contiguous run of the (network, partition) list. */
+std::vector<WorkItem>
+Simulation::xcdWorkTable(const std::vector<std::pair<uint32_t, uint32_t> >& items)
+{
+	const size_t G = items.size();
+	const size_t X = 8;
+	std::vector<WorkItem> work(G);
+	std::vector<size_t> start(X + 1, 0);
+	for(size_t x = 0; x < X; ++x) {
+		size_t cnt = G / X + ((x < G % X) ? 1 : 0);
+		start[x + 1] = start[x] + cnt;
+	}
+	for(size_t b = 0; b < G; ++b) {
+		size_t x = b % X, j = b / X;
+		const auto& it = items[start[x] + j];
+		work[b].net = it.first;
+		work[b].part = it.second;
+	}
+	return work;
+}
+
+static uint32_t
+roundUp64(uint32_t v)
+{
+	return (v + 63u) & ~63u;
+}
+
+Simulation::Simulation(const NetworkImpl& net, const ConfigurationImpl& conf) :
+	m_stdp(conf.stdp),
+	m_writeOnly(conf.writeOnlySynapses),
+	m_device(conf.device < 0 ? 0 : conf.device)
+{
+	std::memset(&m_sp, 0, sizeof(m_sp));
+	hipCheck(hipSetDevice(m_device), "hipSetDevice");
+	hipStream_t st;
+	hipCheck(hipStreamCreate(&st), "hipStreamCreate");
+	m_stream = st;
+
+	/* ---- simulation-local indices: types in order, insertion order within a
+	 * type (src/nemo/cpu/Simulation.cpp:58-73, src/nemo/cpu/Neurons.cpp:26-39) */
+	unsigned maxG = 0;
+	m_minG = ~0u;
+	for(const NeuronGroup& g : net.groups) {
+		for(unsigned gi : g.gidx) {
+			m_localToGlobal.push_back(gi);
+			m_model.push_back(uint8_t(g.model->id));
+			maxG = std::max(maxG, gi);
+			m_minG = std::min(m_minG, gi);
+		}
+	}
+	m_n = uint32_t(m_localToGlobal.size());
+	if(m_n >= (1u << 26)) {
+		throw exception(NEMO_API_UNSUPPORTED, "the HIP backend supports up to 2^26 neurons per simulation");
+	}
+	if(m_n > 0) {
+		m_dense.assign(size_t(maxG - m_minG) + 1, 0u);
+		for(uint32_t l = 0; l < m_n; ++l) {
+			m_dense[m_localToGlobal[l] - m_minG] = l + 1;
+		}
+	} else {
+		m_minG = 0;
+	}
+
+	/* ---- delays and the STDP window (src/nemo/StdpFunction.cpp:52-60) */
+	unsigned maxDelay = net.maxDelay();
+	if(maxDelay > 64) {
+		throw exception(NEMO_INVALID_INPUT, fmt("The network has synapses with delay ", maxDelay,
+					"ms. The HIP backend supports a maximum of 64 ms"));
+	}
+	if(m_stdp && maxDelay + conf.postfire.size() > 64) {
+		throw exception(NEMO_INVALID_INPUT, "Max delay + postfire part of STDP function must be <= 64");
+	}
+	m_D = std::max(1u, maxDelay);
+
+	/* ---- STDP tables (src/nemo/StdpProcess.cpp:20-46, src/nemo/StdpFunction.cpp:73-108) */
+	if(m_stdp) {
+		m_sp.minE = fx_toFix(conf.minE);
+		m_sp.maxE = fx_toFix(conf.maxE);
+		m_sp.minI = fx_toFix(conf.minI);
+		m_sp.maxI = fx_toFix(conf.maxI);
+		for(size_t i = 0; i < conf.prefire.size(); ++i) m_sp.fnPre[i] = fx_toFix(conf.prefire[i]);
+		for(size_t i = 0; i < conf.postfire.size(); ++i) m_sp.fnPost[i] = fx_toFix(conf.postfire[i]);
+		uint64_t pot = 0, dep = 0;
+		int bit = 0;
+		for(auto f = conf.postfire.rbegin(); f != conf.postfire.rend(); ++f, ++bit) {
+			if(*f > 0.0f) pot |= uint64_t(1) << bit;
+			if(*f < 0.0f) dep |= uint64_t(1) << bit;
+		}
+		for(auto f = conf.prefire.begin(); f != conf.prefire.end(); ++f, ++bit) {
+			if(*f > 0.0f) pot |= uint64_t(1) << bit;
+			if(*f < 0.0f) dep |= uint64_t(1) << bit;
+		}
+		m_sp.potentiation = pot;
+		m_sp.depression = dep;
+		m_sp.preLen = int32_t(conf.prefire.size());
+		m_sp.postLen = int32_t(conf.postfire.size());
+		uint64_t preMask = m_sp.preLen >= 64 ? ~uint64_t(0) : ~(~uint64_t(0) << m_sp.preLen);
+		m_sp.prePost = preMask << m_sp.postLen;
+		m_sp.postPre = m_sp.postLen >= 64 ? ~uint64_t(0) : ~(~uint64_t(0) << m_sp.postLen);
+	}
+
+	/* ---- partition geometry */
+	int dev = m_device;
+	int ldsMax = 0;
+	hipCheck(hipDeviceGetAttribute(&ldsMax, hipDeviceAttributeMaxSharedMemoryPerBlock, dev), "hipDeviceGetAttribute");
+	const uint32_t ldsLimit = std::min<uint32_t>(LDS_LIMIT, uint32_t(ldsMax));
+
+	/* ---- synapse validation and saturation bound */
+	const std::vector<unsigned>& S = net.src();
+	const std::vector<unsigned>& Tg = net.tgt();
+	const std::vector<unsigned char>& Dl = net.delay();
+	const std::vector<float>& W = net.weight();
+	const std::vector<unsigned char>& Pl = net.plastic();
+	m_nsyn = S.size();
+	std::vector<uint32_t> lsrc(m_nsyn), ltgt(m_nsyn);
+	std::vector<int32_t> wfx(m_nsyn);
+	std::vector<int64_t> boundE(m_n, 0), boundI(m_n, 0);
+	for(size_t i = 0; i < m_nsyn; ++i) {
+		unsigned s = S[i], t = Tg[i];
+		uint32_t ls = (s >= m_minG && size_t(s - m_minG) < m_dense.size()) ? m_dense[s - m_minG] : 0u;
+		if(!ls) {
+			throw exception(NEMO_INVALID_INPUT, fmt("Invalid synapse source neuron ", s));
+		}
+		uint32_t lt = (t >= m_minG && size_t(t - m_minG) < m_dense.size()) ? m_dense[t - m_minG] : 0u;
+		if(!lt) {
+			throw exception(NEMO_INVALID_INPUT, fmt("Invalid synapse target neuron ", t, " (source: ", s, ")"));
+		}
+		lsrc[i] = ls - 1;
+		ltgt[i] = lt - 1;
+		int32_t w = fx_toFix(W[i]);   /* src/nemo/ConnectivityMatrix.cpp:101 */
+		wfx[i] = w;
+		int64_t wb = w;
+		if(m_stdp && Pl[i]) {
+			if(w > 0) wb = std::max<int64_t>(w, m_sp.maxE);
+			else if(w < 0) wb = std::min<int64_t>(w, m_sp.maxI);
+		}
+		if(w >= 0) boundE[lt - 1] += wb; else boundI[lt - 1] += wb;
+		if(m_stdp && Pl[i]) m_nplastic += 1;
+	}
+	for(uint32_t l = 0; l < m_n; ++l) {
+		if(boundE[l] > int64_t(INT32_MAX) || boundI[l] < int64_t(INT32_MIN)) {
+			m_saturate = true;
+			break;
+		}
+	}
+
+	/* T: neurons per partition. Automatic choice aims at one partition per
+	 * compute unit; the LDS ring [D][T] bounds it from above. */
+	uint32_t T;
+	if(conf.partitionSize) {
+		T = std::min(1024u, roundUp64(conf.partitionSize));
+	} else {
+		T = roundUp64(std::max<uint32_t>(1u, (m_n + 255u) / 256u));
+		T = std::max(64u, std::min(1024u, T));
+	}
+	auto ldsFor = [&](uint32_t t) {
+		uint32_t p = (m_n + t - 1) / t;
+		return nemo_step_lds_bytes(t, m_D, std::max(1u, p), m_saturate ? 1 : 0);
+	};
+	while(T > 64 && ldsFor(T) > ldsLimit) {
+		T -= 64;
+	}
+	if(ldsFor(T) > ldsLimit) {
+		throw exception(NEMO_API_UNSUPPORTED, "maximum delay too large for the LDS delay ring");
+	}
+	m_T = T;
+	m_P = (m_n + T - 1) / T;
+	m_words = m_P * (T / 64u);
+	m_lds = ldsFor(T);
+	if(m_P > 0) {
+		uint32_t maxBlocks = 0;
+		hipCheck(nemo_dev_step_max_blocks(m_T, m_lds, &maxBlocks), "occupancy query");
+		if(m_P > maxBlocks) {
+			throw exception(NEMO_API_UNSUPPORTED, fmt("network of ", m_n, " neurons needs ", m_P,
+						" co-resident workgroups of ", m_T, " neurons; the device holds ", maxBlocks));
+		}
+	}
+
+	/* ---- forward matrix, bucketed by (source, target partition) */
+	const uint32_t P1 = m_P + 1;
+	std::vector<uint32_t> off(size_t(m_n) * P1 + 1, 0u);
+	for(size_t i = 0; i < m_nsyn; ++i) {
+		off[size_t(lsrc[i]) * P1 + ltgt[i] / T] += 1;
+	}
+	uint64_t run = 0;
+	for(size_t k = 0; k < off.size(); ++k) {
+		uint64_t c = off[k];
+		off[k] = uint32_t(run);
+		run += c;
+	}
+	if(run >= (uint64_t(1) << 32)) {
+		throw exception(NEMO_API_UNSUPPORTED, "more than 2^32 synapses in one simulation");
+	}
+	std::vector<uint32_t> cursor(off.begin(), off.end());
+	std::vector<uint32_t> syn(2 * m_nsyn);
+	std::vector<uint32_t> fwd(m_nsyn);
+	for(size_t i = 0; i < m_nsyn; ++i) {
+		uint32_t lt = ltgt[i], p = lt / T;
+		uint32_t pos = cursor[size_t(lsrc[i]) * P1 + p]++;
+		syn[2 * size_t(pos)] = (lt - p * T) | (uint32_t(Dl[i]) << 16);
+		syn[2 * size_t(pos) + 1] = uint32_t(wfx[i]);
+		fwd[i] = pos;
+	}
+	std::vector<uint32_t>().swap(cursor);
+	std::vector<int32_t>().swap(wfx);
+
+	/* ---- synapse query tables, ordinal order per source */
+	if(!m_writeOnly) {
+		m_qStart.assign(size_t(m_n) + 1, 0);
+		for(size_t i = 0; i < m_nsyn; ++i) m_qStart[lsrc[i] + 1] += 1;
+		for(uint32_t l = 0; l < m_n; ++l) m_qStart[l + 1] += m_qStart[l];
+		std::vector<uint64_t> qc(m_qStart.begin(), m_qStart.end() - 1);
+		m_qTarget.resize(m_nsyn);
+		m_qDelay.resize(m_nsyn);
+		m_qPlastic.resize(m_nsyn);
+		m_qFwd.resize(m_nsyn);
+		for(size_t i = 0; i < m_nsyn; ++i) {
+			uint64_t q = qc[lsrc[i]]++;
+			m_qTarget[q] = Tg[i];
+			m_qDelay[q] = Dl[i];
+			m_qPlastic[q] = Pl[i];
+			m_qFwd[q] = fwd[i];
+		}
+	}
+
+	/* ---- reverse matrix of plastic synapses (src/nemo/construction/RCM.cpp:150-171) */
+	std::vector<uint32_t> rcmStart, rcmSd, rcmFwd;
+	if(m_stdp) {
+		rcmStart.assign(size_t(m_n) + 1, 0);
+		for(size_t i = 0; i < m_nsyn; ++i) if(Pl[i]) rcmStart[ltgt[i] + 1] += 1;
+		for(uint32_t l = 0; l < m_n; ++l) rcmStart[l + 1] += rcmStart[l];
+		std::vector<uint32_t> rc(rcmStart.begin(), rcmStart.end() - 1);
+		rcmSd.resize(m_nplastic);
+		rcmFwd.resize(m_nplastic);
+		for(size_t i = 0; i < m_nsyn; ++i) {
+			if(!Pl[i]) continue;
+			uint32_t r = rc[ltgt[i]]++;
+			rcmSd[r] = lsrc[i] | (uint32_t(Dl[i] - 1) << 26);
+			rcmFwd[r] = fwd[i];
+		}
+	}
+	std::vector<uint32_t>().swap(lsrc);
+	std::vector<uint32_t>().swap(ltgt);
+	std::vector<uint32_t>().swap(fwd);
+
+	/* ---- neuron arrays */
+	std::vector<float> param(size_t(nemo_dev::MAX_PARAM) * m_n, 0.0f);
+	std::vector<float> state(size_t(nemo_dev::MAX_STATE) * m_n, 0.0f);
+	{
+		uint32_t l = 0;
+		for(const NeuronGroup& g : net.groups) {
+			for(size_t i = 0; i < g.gidx.size(); ++i, ++l) {
+				for(unsigned k = 0; k < g.model->nparam; ++k) param[size_t(k) * m_n + l] = g.param[k][i];
+				for(unsigned k = 0; k < g.model->nstate; ++k) state[size_t(k) * m_n + l] = g.state[k][i];
+			}
+		}
+	}
+	/* per-neuron RNG seeds, src/nemo/RNG.cpp:38-63: neuron k takes draws
+	 * 4k .. 4k+3 of the default-seeded MT19937, each mapped to draw >> 1 */
+	std::vector<uint32_t> rng(size_t(4) * m_n);
+	{
+		std::mt19937 mt;
+		for(uint32_t l = 0; l < m_n; ++l) {
+			for(unsigned k = 0; k < 4; ++k) rng[size_t(k) * m_n + l] = uint32_t(mt()) >> 1;
+		}
+	}
+
+	/* ---- upload */
+	d_model.upload(m_model.data(), m_model.size());
+	d_param.upload(param.data(), param.size() * sizeof(float));
+	d_state.upload(state.data(), state.size() * sizeof(float));
+	d_rng.upload(rng.data(), rng.size() * sizeof(uint32_t));
+	d_recent.alloc(size_t(m_n) * 8);
+	d_ring.alloc(size_t(m_P) * 2 * m_D * m_T * 4);
+	d_off.upload(off.data(), off.size() * sizeof(uint32_t));
+	d_syn.upload(syn.data(), syn.size() * sizeof(uint32_t));
+	d_fl.alloc(size_t(2) * m_P * m_T * 4);
+	d_fc.alloc(size_t(2) * m_P * 4 + 4);
+	d_ctl.alloc(64);
+	if(m_stdp) {
+		d_rfg.alloc(size_t(2) * m_n * 8);
+		d_rcmStart.upload(rcmStart.data(), rcmStart.size() * 4);
+		d_rcmSd.upload(rcmSd.data(), rcmSd.size() * 4);
+		d_rcmFwd.upload(rcmFwd.data(), rcmFwd.size() * 4);
+		d_rcmAcc.alloc(std::max<size_t>(rcmSd.size(), 1) * 4);
+		d_stdp.upload(&m_sp, sizeof(m_sp));
+	}
+	hipStream_t s = static_cast<hipStream_t>(m_stream);
+	if(d_recent.bytes) hipCheck(nemo_dev_fill_u64(d_recent.as<uint64_t>(), 0, m_n, s), "fill");
+	if(d_ring.bytes) hipCheck(nemo_dev_fill_u32(d_ring.as<uint32_t>(), 0, d_ring.bytes / 4, s), "fill");
+	if(d_rfg.bytes) hipCheck(nemo_dev_fill_u64(d_rfg.as<uint64_t>(), 0, size_t(2) * m_n, s), "fill");
+	if(d_rcmAcc.bytes) hipCheck(nemo_dev_fill_u32(d_rcmAcc.as<uint32_t>(), 0, d_rcmAcc.bytes / 4, s), "fill");
+	hipCheck(nemo_dev_fill_u32(d_ctl.as<uint32_t>(), 0, 16, s), "fill");
+
+	std::vector<std::pair<uint32_t, uint32_t> > items;
+	for(uint32_t p = 0; p < m_P; ++p) items.push_back(std::make_pair(0u, p));
+	std::vector<WorkItem> work = xcdWorkTable(items);
+	d_work.upload(work.data(), work.size() * sizeof(WorkItem));
+	d_net.alloc(sizeof(NetDev));
+	hipCheck(hipStreamSynchronize(s), "hipStreamSynchronize");
+
+	m_deviceBytes = d_model.bytes + d_param.bytes + d_state.bytes + d_rng.bytes + d_recent.bytes
+		+ d_ring.bytes + d_off.bytes + d_syn.bytes + d_fl.bytes + d_fc.bytes + d_rfg.bytes
+		+ d_rcmStart.bytes + d_rcmSd.bytes + d_rcmFwd.bytes + d_rcmAcc.bytes;
+
+	if(conf.logging) {
+		fprintf(stdout, "nemosim HIP: %u neurons, %zu synapses, %u partitions x %u, max delay %u, "
+				"%s accumulation, %.1f MB device\n", m_n, m_nsyn, m_P, m_T, m_D,
+				m_saturate ? "saturating" : "int32", double(m_deviceBytes) / 1e6);
+	}
+	resetTimer();
+}
+
+Simulation::~Simulation()
+{
+	if(m_stream) {
+		(void) hipStreamSynchronize(static_cast<hipStream_t>(m_stream));
+		(void) hipStreamDestroy(static_cast<hipStream_t>(m_stream));
+	}
+}
+
+unsigned
+Simulation::localIdx(unsigned g) const
+{
+	if(g >= m_minG && size_t(g - m_minG) < m_dense.size()) {
+		uint32_t l = m_dense[g - m_minG];
+		if(l) return l - 1;
+	}
+	throw exception(NEMO_INVALID_INPUT, fmt("Non-existing neuron index ", g));
+}
+
+/* Stimulus schedule to device CSR in simulation-local indices; forced
+ * firing is idempotent and a current assignment keeps the last value per
+ * neuron and cycle (src/nemo/cpu/Simulation.cpp:140-163). */
+void
+Simulation::uploadSchedule(unsigned nsteps, const nemo_schedule_t* s)
+{
+	m_hasSchedule = false;
+	if(!s) return;
+	std::vector<uint32_t> fsPtr, fsIdx, isPtr, isIdx;
+	std::vector<float> isVal;
+	bool anyF = s->fstim_ptr && s->fstim_idx;
+	bool anyI = s->istim_ptr && s->istim_idx && s->istim_val;
+	if(!anyF && !anyI) return;
+	fsPtr.assign(size_t(nsteps) + 1, 0);
+	isPtr.assign(size_t(nsteps) + 1, 0);
+	std::unordered_map<uint32_t, size_t> last;
+	for(unsigned k = 0; k < nsteps; ++k) {
+		if(anyF) {
+			for(unsigned j = s->fstim_ptr[k]; j < s->fstim_ptr[k + 1]; ++j) {
+				fsIdx.push_back(localIdx(s->fstim_idx[j]));
+			}
+		}
+		fsPtr[k + 1] = uint32_t(fsIdx.size());
+		if(anyI) {
+			last.clear();
+			size_t base = isIdx.size();
+			for(unsigned j = s->istim_ptr[k]; j < s->istim_ptr[k + 1]; ++j) {
+				uint32_t l = localIdx(s->istim_idx[j]);
+				auto it = last.find(l);
+				if(it == last.end()) {
+					last[l] = isIdx.size();
+					isIdx.push_back(l);
+					isVal.push_back(s->istim_val[j]);
+				} else {
+					isVal[it->second] = s->istim_val[j];
+				}
+			}
+			(void) base;
+		}
+		isPtr[k + 1] = uint32_t(isIdx.size());
+	}
+	d_fsPtr.ensure(fsPtr.size() * 4);
+	d_isPtr.ensure(isPtr.size() * 4);
+	d_fsIdx.ensure(std::max<size_t>(fsIdx.size(), 1) * 4);
+	d_isIdx.ensure(std::max<size_t>(isIdx.size(), 1) * 4);
+	d_isVal.ensure(std::max<size_t>(isVal.size(), 1) * 4);
+	hipStream_t st = static_cast<hipStream_t>(m_stream);
+	hipCheck(hipMemcpyAsync(d_fsPtr.ptr, fsPtr.data(), fsPtr.size() * 4, hipMemcpyHostToDevice, st), "H2D");
+	hipCheck(hipMemcpyAsync(d_isPtr.ptr, isPtr.data(), isPtr.size() * 4, hipMemcpyHostToDevice, st), "H2D");
+	if(!fsIdx.empty()) hipCheck(hipMemcpyAsync(d_fsIdx.ptr, fsIdx.data(), fsIdx.size() * 4, hipMemcpyHostToDevice, st), "H2D");
+	if(!isIdx.empty()) {
+		hipCheck(hipMemcpyAsync(d_isIdx.ptr, isIdx.data(), isIdx.size() * 4, hipMemcpyHostToDevice, st), "H2D");
+		hipCheck(hipMemcpyAsync(d_isVal.ptr, isVal.data(), isVal.size() * 4, hipMemcpyHostToDevice, st), "H2D");
+	}
+	/* the host vectors go out of scope: wait for the copies */
+	hipCheck(hipStreamSynchronize(st), "hipStreamSynchronize");
+	m_hasSchedule = true;
+}
+
+NetDev
+Simulation::prepareLaunch(unsigned nsteps, void* stream)
+{
+	hipCheck(hipSetDevice(m_device), "hipSetDevice");
+	hipStream_t st = static_cast<hipStream_t>(stream);
+	d_raster.ensure(std::max<size_t>(size_t(nsteps) * m_words, 1) * 8);
+	hipCheck(nemo_dev_fill_u32(d_ctl.as<uint32_t>(), 0, 16, st), "fill");
+	NetDev nd;
+	std::memset(&nd, 0, sizeof(nd));
+	nd.n = m_n;
+	nd.P = m_P;
+	nd.T = m_T;
+	nd.D = m_D;
+	nd.words = m_words;
+	nd.flags = (m_stdp ? NEMO_NET_STDP : 0u) | (m_saturate ? NEMO_NET_SATURATE : 0u);
+	nd.nsteps = nsteps;
+	nd.spinLimit = m_spinLimit;
+	nd.cycle0 = m_cycle;
+	nd.model = d_model.as<uint8_t>();
+	nd.param = d_param.as<float>();
+	nd.state = d_state.as<float>();
+	nd.rng = d_rng.as<uint32_t>();
+	nd.recent = d_recent.as<uint64_t>();
+	nd.ring = d_ring.as<int32_t>();
+	nd.off = d_off.as<uint32_t>();
+	nd.syn = d_syn.as<uint32_t>();
+	nd.fl = d_fl.as<uint32_t>();
+	nd.fc = d_fc.as<uint32_t>();
+	nd.rfg = d_rfg.as<uint64_t>();
+	nd.bar = d_ctl.as<uint32_t>();
+	nd.status = d_ctl.as<uint32_t>() + 1;
+	nd.events = reinterpret_cast<unsigned long long*>(d_ctl.as<uint32_t>() + 2);
+	nd.rcm_start = d_rcmStart.as<uint32_t>();
+	nd.rcm_sd = d_rcmSd.as<uint32_t>();
+	nd.rcm_fwd = d_rcmFwd.as<uint32_t>();
+	nd.rcm_acc = d_rcmAcc.as<int32_t>();
+	nd.stdp = d_stdp.ptr;
+	if(m_hasSchedule) {
+		nd.fs_ptr = d_fsPtr.as<uint32_t>();
+		nd.fs_idx = d_fsIdx.as<uint32_t>();
+		nd.is_ptr = d_isPtr.as<uint32_t>();
+		nd.is_idx = d_isIdx.as<uint32_t>();
+		nd.is_val = d_isVal.as<float>();
+	}
+	nd.raster = d_raster.as<uint64_t>();
+	return nd;
+}
+
+void
+Simulation::finishLaunch(unsigned nsteps, void* stream, unsigned long long* events)
+{
+	hipStream_t st = static_cast<hipStream_t>(stream);
+	uint32_t ctl[4];
+	hipCheck(hipMemcpyAsync(ctl, d_ctl.ptr, sizeof(ctl), hipMemcpyDeviceToHost, st), "D2H");
+	hipCheck(hipStreamSynchronize(st), "hipStreamSynchronize");
+	m_cycle += nsteps;
+	m_timerCycles += nsteps;
+	m_lastRunSteps = nsteps;
+	m_hasSchedule = false;
+	if(ctl[1] & NEMO_STATUS_BARRIER_TIMEOUT) {
+		throw exception(NEMO_CUDA_ERROR, "step kernel: a workgroup barrier timed out (workgroups not co-resident?)");
+	}
+	if(events) {
+		unsigned long long e;
+		std::memcpy(&e, &ctl[2], sizeof(e));
+		*events = e;
+	}
+}
+
+void
+Simulation::launch(unsigned nsteps, unsigned long long* events)
+{
+	if(events) *events = 0;
+	if(m_P == 0 || nsteps == 0) {
+		m_cycle += nsteps;
+		m_timerCycles += nsteps;
+		m_lastRunSteps = nsteps;
+		m_hasSchedule = false;
+		return;
+	}
+	hipStream_t st = static_cast<hipStream_t>(m_stream);
+	NetDev nd = prepareLaunch(nsteps, m_stream);
+	hipCheck(hipMemcpyAsync(d_net.ptr, &nd, sizeof(nd), hipMemcpyHostToDevice, st), "H2D");
+	/* HIP events on the kernel's own stream time the step kernel alone */
+	hipEvent_t e0, e1;
+	hipCheck(hipEventCreate(&e0), "hipEventCreate");
+	hipCheck(hipEventCreate(&e1), "hipEventCreate");
+	hipCheck(hipEventRecord(e0, st), "hipEventRecord");
+	hipCheck(nemo_dev_step_launch(d_net.as<NetDev>(), d_work.as<WorkItem>(), m_P, m_T, m_lds, st),
+			"step kernel launch");
+	hipCheck(hipEventRecord(e1, st), "hipEventRecord");
+	finishLaunch(nsteps, m_stream, events);
+	float ms = 0.0f;
+	hipCheck(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+	m_kernelMs = ms;
+	(void) hipEventDestroy(e0);
+	(void) hipEventDestroy(e1);
+}
+
+void
+Simulation::decodeStep(const uint64_t* words, std::vector<unsigned>& out) const
+{
+	out.clear();
+	for(uint32_t w = 0; w < m_words; ++w) {
+		uint64_t b = words[w];
+		while(b) {
+			uint32_t l = w * 64u + uint32_t(__builtin_ctzll(b));
+			b &= b - 1;
+			if(l < m_n) out.push_back(m_localToGlobal[l]);
+		}
+	}
+}
+
+const std::vector<unsigned>&
+Simulation::step(const std::vector<unsigned>& fstim, const unsigned* istimIdx, const float* istimVal, size_t nistim)
+{
+	unsigned fp[2] = { 0, unsigned(fstim.size()) };
+	unsigned ip[2] = { 0, unsigned(nistim) };
+	nemo_schedule_t s;
+	s.fstim_ptr = fp;
+	s.fstim_idx = fstim.empty() ? nullptr : fstim.data();
+	s.istim_ptr = ip;
+	s.istim_idx = istimIdx;
+	s.istim_val = istimVal;
+	if(fstim.empty()) { s.fstim_ptr = nullptr; }
+	if(nistim == 0) { s.istim_ptr = nullptr; s.istim_idx = nullptr; s.istim_val = nullptr; }
+	uploadSchedule(1, &s);
+	launch(1, nullptr);
+	m_rasterHost.resize(m_words);
+	if(m_words) {
+		hipCheck(hipMemcpy(m_rasterHost.data(), d_raster.ptr, size_t(m_words) * 8, hipMemcpyDeviceToHost), "D2H");
+	}
+	decodeStep(m_rasterHost.data(), m_fired);
+	return m_fired;
+}
+
+void
+Simulation::run(unsigned nsteps, const nemo_schedule_t* schedule, unsigned long long* events)
+{
+	uploadSchedule(nsteps, schedule);
+	launch(nsteps, events);
+}
+
+void
+Simulation::readFired(unsigned counts[], unsigned ids[], size_t cap, size_t* total)
+{
+	size_t nw = size_t(m_lastRunSteps) * m_words;
+	m_rasterHost.resize(nw);
+	if(nw) {
+		hipCheck(hipMemcpy(m_rasterHost.data(), d_raster.ptr, nw * 8, hipMemcpyDeviceToHost), "D2H");
+	}
+	size_t k = 0;
+	std::vector<unsigned> one;
+	for(unsigned t = 0; t < m_lastRunSteps; ++t) {
+		decodeStep(m_rasterHost.data() + size_t(t) * m_words, one);
+		if(counts) counts[t] = unsigned(one.size());
+		if(ids) {
+			if(k + one.size() > cap) {
+				throw exception(NEMO_BUFFER_OVERFLOW, "fired-id buffer too small");
+			}
+			std::copy(one.begin(), one.end(), ids + k);
+		}
+		k += one.size();
+	}
+	if(total) *total = k;
+}
+
+void
+Simulation::readRaster(uint64_t words[], size_t cap, size_t* wordsPerStep)
+{
+	size_t nw = size_t(m_lastRunSteps) * m_words;
+	if(wordsPerStep) *wordsPerStep = m_words;
+	if(words) {
+		if(cap < nw) throw exception(NEMO_BUFFER_OVERFLOW, "raster buffer too small");
+		if(nw) hipCheck(hipMemcpy(words, d_raster.ptr, nw * 8, hipMemcpyDeviceToHost), "D2H");
+	}
+}
+
+void
+Simulation::info(nemo_hip_info_t* out) const
+{
+	out->neurons = m_n;
+	out->partitions = m_P;
+	out->partition_size = m_T;
+	out->max_delay = m_D;
+	out->saturating = m_saturate ? 1 : 0;
+	out->stdp = m_stdp ? 1 : 0;
+	out->synapses = m_nsyn;
+	out->plastic_synapses = m_nplastic;
+	out->device_bytes = m_deviceBytes;
+	out->lds_bytes = m_lds;
+	out->block_threads = m_T;
+	out->last_kernel_ms = m_kernelMs;
+}
+
+/* src/nemo/ConnectivityMatrix.cpp:241-294 */
+void
+Simulation::applyStdp(float reward)
+{
+	if(!m_stdp) {
+		throw exception(NEMO_LOGIC_ERROR, "applyStdp called, but no STDP model specified");
+	}
+	int32_t fxr = fx_toFix(reward);
+	if(fxr == 0 && reward != 0) {
+		throw exception(NEMO_INVALID_INPUT, fmt("STDP reward rounded down to zero. The smallest valid reward is ",
+					fx_toFloat(1)));
+	}
+	hipCheck(hipSetDevice(m_device), "hipSetDevice");
+	hipStream_t st = static_cast<hipStream_t>(m_stream);
+	hipCheck(nemo_dev_apply_stdp(d_syn.as<uint32_t>(), d_rcmFwd.as<uint32_t>(), d_rcmAcc.as<int32_t>(),
+				m_nplastic, d_stdp.ptr, fxr, st), "applyStdp kernel");
+	hipCheck(hipStreamSynchronize(st), "hipStreamSynchronize");
+}
+
+void
+Simulation::readScalar(const DevBuf& b, size_t offset, void* out, size_t bytes) const
+{
+	hipCheck(hipSetDevice(m_device), "hipSetDevice");
+	hipCheck(hipMemcpy(out, static_cast<const char*>(b.ptr) + offset, bytes, hipMemcpyDeviceToHost), "D2H");
+}
+
+void
+Simulation::writeScalar(DevBuf& b, size_t offset, const void* in, size_t bytes)
+{
+	hipCheck(hipSetDevice(m_device), "hipSetDevice");
+	hipCheck(hipMemcpy(static_cast<char*>(b.ptr) + offset, in, bytes, hipMemcpyHostToDevice), "H2D");
+}
+
+float
+Simulation::getNeuronState(unsigned g, unsigned var) const
+{
+	unsigned l = localIdx(g);
+	if(var >= modelInfo(m_model[l]).nstate) {
+		throw exception(NEMO_INVALID_INPUT, fmt("Invalid state variable index ", var));
+	}
+	float v;
+	readScalar(d_state, (size_t(var) * m_n + l) * 4, &v, 4);
+	return v;
+}
+
+float
+Simulation::getNeuronParameter(unsigned g, unsigned param) const
+{
+	unsigned l = localIdx(g);
+	if(param >= modelInfo(m_model[l]).nparam) {
+		throw exception(NEMO_INVALID_INPUT, fmt("Invalid parameter index ", param));
+	}
+	float v;
+	readScalar(d_param, (size_t(param) * m_n + l) * 4, &v, 4);
+	return v;
+}
+
+float
+Simulation::getMembranePotential(unsigned g) const
+{
+	unsigned l = localIdx(g);
+	return getNeuronState(g, modelInfo(m_model[l]).membranePotential);
+}
+
+void
+Simulation::setNeuron(unsigned g, unsigned nargs, const float args[])
+{
+	unsigned l = localIdx(g);
+	const ModelInfo& m = modelInfo(m_model[l]);
+	if(nargs != m.nparam + m.nstate) {
+		throw exception(NEMO_INVALID_INPUT, fmt("Unexpected number of parameters/state variables when modifying neuron. Expected ",
+					m.nparam + m.nstate, ", found ", nargs));
+	}
+	for(unsigned k = 0; k < m.nparam; ++k) writeScalar(d_param, (size_t(k) * m_n + l) * 4, &args[k], 4);
+	for(unsigned k = 0; k < m.nstate; ++k) writeScalar(d_state, (size_t(k) * m_n + l) * 4, &args[m.nparam + k], 4);
+}
+
+void
+Simulation::setNeuronState(unsigned g, unsigned var, float val)
+{
+	unsigned l = localIdx(g);
+	if(var >= modelInfo(m_model[l]).nstate) {
+		throw exception(NEMO_INVALID_INPUT, fmt("Invalid state variable index ", var));
+	}
+	writeScalar(d_state, (size_t(var) * m_n + l) * 4, &val, 4);
+}
+
+void
+Simulation::setNeuronParameter(unsigned g, unsigned param, float val)
+{
+	unsigned l = localIdx(g);
+	if(param >= modelInfo(m_model[l]).nparam) {
+		throw exception(NEMO_INVALID_INPUT, fmt("Invalid parameter index ", param));
+	}
+	writeScalar(d_param, (size_t(param) * m_n + l) * 4, &val, 4);
+}
+
+size_t
+Simulation::synapseRecord(synapse_id id) const
+{
+	if(m_writeOnly) {
+		throw exception(NEMO_INVALID_INPUT, "Cannot read synapse state if simulation configured with write-only synapses");
+	}
+	unsigned source = unsigned(id >> 32);
+	size_t k = size_t(id & 0xffffffffu);
+	unsigned l;
+	try {
+		l = localIdx(source);
+	} catch(exception&) {
+		throw exception(NEMO_INVALID_INPUT, fmt("Non-existing neuron id (", source, ") in synapse query"));
+	}
+	if(k >= m_qStart[l + 1] - m_qStart[l]) {
+		throw exception(NEMO_INVALID_INPUT, "synapse id out or range");
+	}
+	return size_t(m_qStart[l] + k);
+}
+
+const std::vector<synapse_id>&
+Simulation::getSynapsesFrom(unsigned source)
+{
+	if(m_writeOnly) {
+		throw exception(NEMO_INVALID_INPUT, "Cannot read synapse state if simulation configured with write-only synapses");
+	}
+	unsigned l;
+	try {
+		l = localIdx(source);
+	} catch(exception&) {
+		throw exception(NEMO_INVALID_INPUT, fmt("Non-existing source neuron id (", source, ") in synapse id query"));
+	}
+	size_t cnt = size_t(m_qStart[l + 1] - m_qStart[l]);
+	m_queried.resize(cnt);
+	for(size_t k = 0; k < cnt; ++k) m_queried[k] = (synapse_id(source) << 32) | synapse_id(k);
+	return m_queried;
+}
+
+unsigned Simulation::getSynapseSource(synapse_id id) const { synapseRecord(id); return unsigned(id >> 32); }
+unsigned Simulation::getSynapseTarget(synapse_id id) const { return m_qTarget[synapseRecord(id)]; }
+unsigned Simulation::getSynapseDelay(synapse_id id) const { return m_qDelay[synapseRecord(id)]; }
+unsigned char Simulation::getSynapsePlastic(synapse_id id) const { return m_qPlastic[synapseRecord(id)]; }
+
+float
+Simulation::getSynapseWeight(synapse_id id) const
+{
+	size_t q = synapseRecord(id);
+	int32_t w;
+	readScalar(d_syn, (size_t(m_qFwd[q]) * 2 + 1) * 4, &w, 4);
+	return fx_toFloat(w);
+}
+
+unsigned long
+Simulation::elapsedWallclock() const
+{
+	auto now = std::chrono::steady_clock::now();
+	return (unsigned long) std::chrono::duration_cast<std::chrono::milliseconds>(now - m_timerStart).count();
+}
+
+void
+Simulation::resetTimer()
+{
+	m_timerStart = std::chrono::steady_clock::now();
+	m_timerCycles = 0;
+}
+
+} // namespace hip
+} // namespace nemo

@@ -1,0 +1,162 @@
+#ifndef NEMOSIM_HOST_SIMULATION_HPP
+#define NEMOSIM_HOST_SIMULATION_HPP
+
+#include <chrono>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "common.hpp"
+#include "network.hpp"
+#include "../device/models.h"
+#include "../device/step_abi.h"
+#include "nemo_hip.h"
+
+namespace nemo {
+namespace hip {
+
+/* Configuration (src/nemo/ConfigurationImpl.hpp) */
+struct ConfigurationImpl {
+	bool stdp = false;
+	std::vector<float> prefire, postfire;
+	float minE = 0, maxE = 0, minI = 0, maxI = 0;
+	int backend = NEMO_BACKEND_CUDA;
+	int device = 0;
+	bool writeOnlySynapses = false;
+	bool logging = false;
+	unsigned partitionSize = 0;   /* 0 = automatic */
+	std::string description;
+
+	/* src/nemo/StdpFunction.cpp:12-50 */
+	void setStdpFunction(const std::vector<float>& pre, const std::vector<float>& post,
+			float minE, float maxE, float minI, float maxI);
+};
+
+/* A device allocation that frees itself */
+struct DevBuf {
+	void* ptr = nullptr;
+	size_t bytes = 0;
+	DevBuf() = default;
+	DevBuf(const DevBuf&) = delete;
+	DevBuf& operator=(const DevBuf&) = delete;
+	~DevBuf();
+	void alloc(size_t b);
+	void ensure(size_t b);   /* grow (contents not kept) */
+	void upload(const void* src, size_t b);
+	template<typename T> T* as() const { return static_cast<T*>(ptr); }
+};
+
+/*
+ * One network on the GPU (the HIP counterpart of NeMo's
+ * cpu::Simulation / cuda::Simulation behind SimulationBackend,
+ * src/nemo/SimulationBackend.hpp:40-142).
+ */
+class Simulation
+{
+	public:
+		Simulation(const NetworkImpl& net, const ConfigurationImpl& conf);
+		~Simulation();
+
+		/* one cycle with stimulus (src/nemo/nemo_c.cpp:440-461); the result is
+		 * valid until the next step */
+		const std::vector<unsigned>& step(const std::vector<unsigned>& fstim,
+				const unsigned* istimIdx, const float* istimVal, size_t nistim);
+
+		void applyStdp(float reward);
+
+		/* multi-cycle runs */
+		void run(unsigned nsteps, const nemo_schedule_t* schedule, unsigned long long* events);
+		void readFired(unsigned counts[], unsigned ids[], size_t cap, size_t* total);
+		void readRaster(uint64_t words[], size_t cap, size_t* wordsPerStep);
+		void info(nemo_hip_info_t* out) const;
+
+		/* queries (src/nemo/Simulation.hpp:130-200) */
+		float getMembranePotential(unsigned g) const;
+		float getNeuronState(unsigned g, unsigned var) const;
+		float getNeuronParameter(unsigned g, unsigned param) const;
+		void setNeuron(unsigned g, unsigned nargs, const float args[]);
+		void setNeuronState(unsigned g, unsigned var, float val);
+		void setNeuronParameter(unsigned g, unsigned param, float val);
+		const std::vector<synapse_id>& getSynapsesFrom(unsigned source);
+		unsigned getSynapseSource(synapse_id id) const;
+		unsigned getSynapseTarget(synapse_id id) const;
+		unsigned getSynapseDelay(synapse_id id) const;
+		float getSynapseWeight(synapse_id id) const;
+		unsigned char getSynapsePlastic(synapse_id id) const;
+
+		unsigned long elapsedWallclock() const;
+		unsigned long elapsedSimulation() const { return (unsigned long) m_timerCycles; }
+		void resetTimer();
+
+		/* batched launches (nemo_hip_run_batch) */
+		uint32_t partitions() const { return m_P; }
+		uint32_t blockThreads() const { return m_T; }
+		uint32_t ldsBytes() const { return m_lds; }
+		/* prepare a launch of nsteps cycles: fills and returns the device
+		 * descriptor (host copy) with per-launch fields set */
+		NetDev prepareLaunch(unsigned nsteps, void* stream);
+		void finishLaunch(unsigned nsteps, void* stream, unsigned long long* events);
+		unsigned neuronCount() const { return m_n; }
+
+		static std::vector<WorkItem> xcdWorkTable(const std::vector<std::pair<uint32_t, uint32_t> >& items);
+
+	private:
+		uint32_t m_n = 0, m_P = 0, m_T = 64, m_D = 1, m_words = 0, m_lds = 0;
+		bool m_saturate = false;
+		bool m_stdp = false;
+		bool m_writeOnly = false;
+		size_t m_nsyn = 0, m_nplastic = 0;
+		uint64_t m_cycle = 0;          /* absolute cycle (ring slot, buffer parity) */
+		uint64_t m_timerCycles = 0;    /* cycles since the last timer reset */
+		std::chrono::steady_clock::time_point m_timerStart;
+		int m_device = 0;
+		void* m_stream = nullptr;
+		uint32_t m_spinLimit = 1u << 22;
+
+		/* index maps */
+		std::vector<unsigned> m_localToGlobal;
+		std::vector<uint32_t> m_dense;         /* gidx - m_minG -> local + 1 (0: none) */
+		unsigned m_minG = 0;
+		std::vector<uint8_t> m_model;
+		std::vector<unsigned> m_typeOfLocal;   /* model id per local neuron */
+
+		/* synapse queries: per local source, flat synapse records in ordinal order */
+		std::vector<uint64_t> m_qStart;
+		std::vector<uint32_t> m_qTarget;       /* global target */
+		std::vector<uint8_t> m_qDelay, m_qPlastic;
+		std::vector<uint32_t> m_qFwd;          /* record index in the forward matrix */
+		std::vector<synapse_id> m_queried;
+
+		nemo_dev::StdpParams m_sp;
+
+		/* device data */
+		DevBuf d_model, d_param, d_state, d_rng, d_recent, d_ring;
+		DevBuf d_off, d_syn;
+		DevBuf d_fl, d_fc, d_rfg, d_ctl;       /* ctl: bar, status, events */
+		DevBuf d_rcmStart, d_rcmSd, d_rcmFwd, d_rcmAcc, d_stdp;
+		DevBuf d_fsPtr, d_fsIdx, d_isPtr, d_isIdx, d_isVal;
+		DevBuf d_raster;
+		DevBuf d_net, d_work;
+		size_t m_deviceBytes = 0;
+		float m_kernelMs = 0.0f;
+
+		/* schedule of the current launch (host side, sim-local indices) */
+		bool m_hasSchedule = false;
+		unsigned m_lastRunSteps = 0;
+
+		std::vector<unsigned> m_fired;
+		std::vector<uint64_t> m_rasterHost;
+
+		unsigned localIdx(unsigned g) const;
+		void uploadSchedule(unsigned nsteps, const nemo_schedule_t* s);
+		void launch(unsigned nsteps, unsigned long long* events);
+		void readScalar(const DevBuf& b, size_t offset, void* out, size_t bytes) const;
+		void writeScalar(DevBuf& b, size_t offset, const void* in, size_t bytes);
+		void decodeStep(const uint64_t* words, std::vector<unsigned>& out) const;
+		size_t synapseRecord(synapse_id id) const;
+};
+
+} // namespace hip
+} // namespace nemo
+
+#endif

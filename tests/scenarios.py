@@ -1,0 +1,233 @@
+"""Closed-form step scenarios from NeMo's own test suite (src/test/test.cpp,
+src/test/stdp.cpp), written once against the builder interface of
+backends.py. Each returns nothing and asserts the reference's expectation;
+test_oracle_scenarios.py runs them on the oracle, test_gpu_scenarios.py on the
+HIP backend."""
+import numpy as np
+
+import netutil
+
+IZ = "Izhikevich"
+IF = "IF_curr_exp"
+
+
+def create_ring(b, ncount, n0=0, delay=1, plastic=0):
+    """src/test/utils.cpp:133-145"""
+    for i in range(ncount):
+        v, bb, r2 = -65.0, 0.2, 0.25
+        src = n0 + i
+        b.add_neuron(IZ, src, 0.02, bb, v + 15.0 * r2, 8.0 - 6.0 * r2, 0.0, np.float32(bb) * np.float32(v), v)
+        b.add_synapse(src, n0 + (i + 1) % ncount, delay, 1000.0, plastic)
+
+
+def ring(make, ncount, delay, partition_size=None):
+    """src/test/test.cpp:299-325"""
+    b = make()
+    create_ring(b, ncount, delay=delay)
+    sim = b.simulation(partition_size)
+    duration = ncount * 5 // 2
+    sim.step([0])
+    for ms in range(1, duration):
+        fired = sim.step()
+        if delay == 1:
+            assert fired == [ms % ncount], (ms, fired)
+        elif ms % delay == 0:
+            assert fired == [(ms // delay) % ncount], (ms, fired)
+        else:
+            assert fired == [], (ms, fired)
+
+
+def double_ring(make, partition_size=None):
+    """src/test/test.cpp:329-364"""
+    ncount = 512
+    b = make()
+    create_ring(b, ncount, 0, delay=1)
+    create_ring(b, ncount, ncount, delay=2)
+    sim = b.simulation(partition_size)
+    sim.step([0, ncount])
+    for ms in range(1, ncount * 5 // 2):
+        fired = sim.step()
+        if ms % 2 == 0:
+            assert fired == [ms % ncount, ncount + (ms // 2) % ncount], (ms, fired)
+        else:
+            assert fired == [ms % ncount], (ms, fired)
+
+
+def add_excitatory_neuron(b, idx, sigma=0.0):
+    """src/test/utils.cpp:107-113"""
+    r = 0.5
+    bb = np.float32(0.25) - np.float32(0.05) * np.float32(r)
+    v = -65.0
+    b.add_neuron(IZ, idx, np.float32(0.02) + np.float32(0.08) * np.float32(r), bb, v, 2.0, sigma,
+                 np.float32(bb) * np.float32(v), v)
+
+
+def firing_stimulus(make, ncount=3000, cycles=300, seed=0):
+    """src/test/test.cpp:205-242: the fired set equals the forced set"""
+    b = make()
+    for n in range(ncount):
+        add_excitatory_neuron(b, n)
+    sim = b.simulation()
+    rng = np.random.default_rng(seed)
+    p = 10.0 / ncount
+    for t in range(cycles):
+        fstim = np.nonzero(rng.random(ncount) < p)[0].tolist()
+        fired = sim.step(fstim)
+        assert sorted(fired) == fstim, t
+
+
+def current_stimulus(make, ncount=1500):
+    """src/test/test.cpp:245-271: injected current starts a ring"""
+    b = make()
+    create_ring(b, ncount)
+    sim = b.simulation()
+    sim.step(None, [(5, 0.001), (8, 0.001), (0, 1000.0), (100, 0.001), (1, 0.001)])
+    for ms in range(1, ncount * 2):
+        fired = sim.step()
+        assert fired == [ms % ncount], (ms, fired)
+
+
+def invalid_current_stimulus(make, error_type):
+    """src/test/test.cpp:274-288"""
+    ncount = 1000
+    b = make()
+    create_ring(b, ncount)
+    sim = b.simulation()
+    sim.step()
+    sim.step()
+    try:
+        sim.step(None, [(ncount + 1, 0.5)])
+    except error_type as e:
+        assert e.status == 6
+        return
+    raise AssertionError("invalid current stimulus did not raise")
+
+
+def fixed_point_saturation(make):
+    """src/test/test.cpp:414-467: a target whose input saturates the Q11.20
+    range behaves like the reference target receiving fx_max."""
+    INPUT0, INPUT1, OUT_SAT, OUT_REF, OUT_WEAK, NCOUNT = range(6)
+    b = make()
+    for n in range(NCOUNT):
+        b.add_neuron(IZ, n, 0.02, 0.2, -65.0, 8.0, 0.0, -13.0, -65.0)
+    b.add_synapse(INPUT0, OUT_REF, 1, 1024.0, 0)
+    b.add_synapse(INPUT1, OUT_REF, 1, 1024.0, 0)
+    b.add_synapse(INPUT0, OUT_WEAK, 1, 1000.0, 0)
+    b.add_synapse(INPUT1, OUT_WEAK, 1, 1000.0, 0)
+    b.add_synapse(INPUT0, OUT_SAT, 1, 1100.0, 0)
+    b.add_synapse(INPUT1, OUT_SAT, 1, 1100.0, 0)
+    sim = b.simulation()
+    assert len(sim.step([INPUT0, INPUT1])) == 2
+    assert len(sim.step()) == 3
+    u_ref = sim.get_neuron_state(OUT_REF, 0)
+    u_weak = sim.get_neuron_state(OUT_WEAK, 0)
+    u_sat = sim.get_neuron_state(OUT_SAT, 0)
+    assert abs(u_ref - u_weak) > 1e-5
+    assert abs(u_ref - u_sat) < 1e-5
+    return u_ref, u_weak, u_sat
+
+
+def integrator_brute_force(make, seed=1, nsrc=24, ntgt=8, cycles=60, dmax=7, saturating=False):
+    """Delivery checked against a brute-force sum: IF_curr_exp targets that
+    integrate their input exactly (tau_syn=1, no leak, no firing) receive
+    spikes from sources forced to fire on a random schedule. Each cycle's
+    E and I input is the Q11.20 sum, computed here with Python integers, of
+    the weights whose (source fired at t-d, delay d) arrive at t; v(t+1) =
+    v(t) + ((E+I)+0) in float32."""
+    rng = np.random.default_rng(seed)
+    b = make()
+    for s in range(nsrc):
+        netutil.if_source(b, s)
+    for t in range(ntgt):
+        netutil.if_integrator(b, nsrc + t)
+    syn = []
+    for s in range(nsrc):
+        for _ in range(int(rng.integers(0, 6))):
+            t = nsrc + int(rng.integers(0, ntgt))
+            d = int(rng.integers(1, dmax + 1))
+            if saturating:
+                w = float(np.float32(rng.uniform(-2047.0, 2047.0)))
+            else:
+                w = float(np.float32(rng.uniform(-1.0, 1.0)))
+            if rng.random() < 0.05:
+                w = 0.0
+            syn.append((s, t, d, w))
+            b.add_synapse(s, t, d, w, 0)
+    sim = b.simulation()
+    sched = [np.nonzero(rng.random(nsrc) < 0.3)[0].tolist() for _ in range(cycles)]
+    v = {t: np.float32(-65.0) for t in range(nsrc, nsrc + ntgt)}
+    for c in range(cycles):
+        fired = sim.step(sched[c])
+        assert sorted(fired) == sched[c], c
+        E = {t: 0 for t in v}
+        I = {t: 0 for t in v}
+        for (s, t, d, w) in syn:
+            if c - d >= 0 and s in sched[c - d]:
+                fx = netutil.fix20(w)
+                if fx >= 0:
+                    E[t] += fx
+                else:
+                    I[t] += fx
+        for t in v:
+            cur = np.float32(netutil.sat_to_float(E[t]) + netutil.sat_to_float(I[t]))
+            cur = np.float32(cur + np.float32(0.0))
+            cur = np.float32(cur + np.float32(0.0))
+            v[t] = np.float32(v[t] + np.float32(np.float32(cur / np.float32(1.0)) + np.float32(0.0)))
+            got = sim.get_neuron_state(t, 0)
+            assert got == v[t], (c, t, got, v[t])
+
+
+def stdp_closed_form(make, reward=1.0, noise=False, epochs=4):
+    """src/test/stdp.cpp:117-265 in miniature: a one-to-one group of
+    pre neurons 0..n-1 projecting to post neurons n..2n-1 with delays
+    1 + (i % 20); both groups forced to fire dt_fire cycles apart; after each
+    epoch + applyStdp the weight equals w0 + epoch*reward*dw(dt) where dw is
+    the window value at the arrival/post-firing offset."""
+    n = 40
+    w0 = 0.5
+    pre_fn = [np.float32(1.0) * np.float32(np.exp(-dt / 20.0)) for dt in range(1, 21)]
+    post_fn = [np.float32(-0.8) * np.float32(np.exp(-dt / 20.0)) for dt in range(1, 21)]
+    b = make()
+    b.set_stdp(pre_fn, post_fn, 0.0, 100.0, 0.0, -100.0)
+    ids = []
+    for i in range(2 * n):
+        b.add_neuron(IZ, i, 0.02, 0.2, -65.0, 8.0, 0.0, -13.0, -65.0)
+    for i in range(n):
+        ids.append(b.add_synapse(i, n + i, 1 + (i % 20), w0, 1))
+    if noise:
+        rng = np.random.default_rng(7)
+        for i in range(n):
+            for _ in range(3):
+                b.add_synapse(i, n + int(rng.integers(0, n)), 1 + int(rng.integers(0, 20)), 0.0, 0)
+    sim = b.simulation()
+    dt_fire = 10   # post fires dt_fire cycles after pre
+    period = 100
+    expect = []
+    for e in range(epochs):
+        for t in range(period):
+            fs = []
+            if t == 10:
+                fs = list(range(n))
+            if t == 10 + dt_fire:
+                fs = list(range(n, 2 * n))
+            sim.step(fs)
+        sim.apply_stdp(reward)
+        expect.append([sim.get_synapse_weight(sid) for sid in ids])
+    # closed form: arrival at 10 + d; post at 20: dt = 20 - (10 + d) = 10 - d
+    fx_reward = netutil.fix20(reward)
+    w = [netutil.fix20(w0)] * n
+    for e in range(epochs):
+        for i in range(n):
+            d = 1 + (i % 20)
+            arrival = 10 + d
+            post = 10 + dt_fire
+            if arrival <= post:
+                dwf = pre_fn[post - arrival]          # pre-post, dt = 0..: fnPre[dt]
+            else:
+                dwf = post_fn[arrival - post - 1]     # post-pre, first cycle after: fnPost[0]
+            acc = netutil.fix20(dwf)
+            delta = (fx_reward * acc) >> 20
+            w[i] = min(netutil.fix20(100.0), max(w[i] + delta, 0))
+        got = expect[e]
+        for i in range(n):
+            assert got[i] == np.float32(np.float32(w[i]) / np.float32(1 << 20)), (e, i, got[i], w[i])

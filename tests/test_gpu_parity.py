@@ -1,0 +1,225 @@
+"""HIP backend vs the oracle on seeded networks: firing rasters bit-exact per
+cycle (first diverging cycle reported), neuron state bit-exact (the FP
+arithmetic keeps NeMo's op order; non-finite values compare by class), and
+synapse weights bit-exact under STDP."""
+import numpy as np
+import pytest
+
+import nrand_domain as dom
+from backends import first_divergence
+from nemosim_amd import examples
+
+pytestmark = pytest.mark.gpu
+
+
+def _hip_sim(nemo_mod, spec, stdp=None, partition_size=None):
+    net = spec.to_nemo(nemo_mod)
+    conf = nemo_mod.Configuration()
+    if stdp is not None:
+        pre, post, min_e, max_e, min_i, max_i = stdp
+        conf.set_stdp_function(pre, post, min_e=min_e, max_e=max_e, min_i=min_i, max_i=max_i)
+    if partition_size:
+        conf.set_partition_size(partition_size)
+    return nemo_mod.Simulation(net, conf)
+
+
+def _same_float(a, b):
+    a = np.asarray(a, np.float32)
+    b = np.asarray(b, np.float32)
+    both_nan = np.isnan(a) & np.isnan(b)
+    return np.all((a.view(np.uint32) == b.view(np.uint32)) | both_nan)
+
+
+def _compare_state(sim, orc, n, nstate, sample=None):
+    idx = range(n) if sample is None else sample
+    for var in range(nstate):
+        ref = orc.state_all(var)
+        got = np.array([sim.get_neuron_state(i, var) for i in idx], np.float32)
+        exp = ref[list(idx)]
+        assert _same_float(got, exp), "state %d differs at neurons %s" % (
+            var, np.nonzero(got.view(np.uint32) != exp.view(np.uint32))[0][:10])
+
+
+def _compare_run(sim, orc, steps):
+    sim.run(steps)
+    gc, gi = sim.read_fired()
+    rc, ri = orc.run(steps)
+    d = first_divergence(rc, ri, gc, gi)
+    assert d is None, "first diverging cycle %d: %s" % d
+    return rc, ri
+
+
+def test_device_libm_exhaustive(nemo_mod, oracle_mod):
+    chunk = 1 << 24
+    for kind in (0, 1):
+        total = dom.LOG_COUNT if kind == 0 else dom.SIN_COUNT
+        bad = 0
+        for start in range(0, total, chunk):
+            cnt = min(chunk, total - start)
+            if kind == 0:
+                x = dom.log_inputs(start, cnt)
+                ref = oracle_mod.libm_logf_grid(start, cnt)
+            else:
+                x = dom.sin_inputs(start, cnt)
+                ref = oracle_mod.libm_sinf(x)
+            got = nemo_mod.eval_libm(kind, x, on_device=True)
+            bad += int(np.count_nonzero(got.view(np.uint32) != ref.view(np.uint32)))
+        assert bad == 0, "kind %d: %d inputs differ from the host C library" % (kind, bad)
+
+
+def test_device_nrand_sequence(nemo_mod, oracle_mod):
+    for state in ([1, 2, 3, 4], [0x7fffffff, 0x12345678, 0x0badf00d, 0x3eadbeef], [5489, 0, 0, 1]):
+        ref = oracle_mod.nrand_seq(state, 4000)
+        got = nemo_mod.nrand_seq(state, 4000)
+        assert _same_float(got, ref)
+
+
+@pytest.mark.parametrize("ps", [None, 64, 256])
+def test_c1_raster_and_state(nemo_mod, oracle_mod, ps):
+    """Izhikevich 80/20, 1000 neurons x 100 synapses, 1000 cycles."""
+    spec = examples.izhikevich_80_20(1000, 100, seed=11)
+    sim = _hip_sim(nemo_mod, spec, partition_size=ps)
+    orc = oracle_mod.from_spec(spec)
+    rc, _ = _compare_run(sim, orc, 1000)
+    assert rc.sum() > 1000
+    _compare_state(sim, orc, 1000, 2)
+
+
+def test_step_api_equals_run_api(nemo_mod, oracle_mod):
+    spec = examples.izhikevich_80_20(1000, 100, seed=5)
+    sim = _hip_sim(nemo_mod, spec)
+    orc = oracle_mod.from_spec(spec)
+    rng = np.random.default_rng(2)
+    for t in range(300):
+        fs = sorted(set(rng.integers(0, 1000, size=int(rng.integers(0, 3))).tolist()))
+        ist = [(int(i), float(np.float32(rng.uniform(0, 20)))) for i in rng.integers(0, 1000, size=int(rng.integers(0, 3)))]
+        a = sim.step(fs, ist)
+        b = orc.step(fs, ist)
+        assert a == b, t
+    # then continue with a device-resident run
+    _compare_run(sim, orc, 500)
+    _compare_state(sim, orc, 1000, 2)
+
+
+def test_run_with_schedule(nemo_mod, oracle_mod):
+    spec = examples.izhikevich_80_20(2000, 100, seed=8)
+    sim = _hip_sim(nemo_mod, spec)
+    orc = oracle_mod.from_spec(spec)
+    rng = np.random.default_rng(4)
+    steps = 200
+    fst = [sorted(set(rng.integers(0, 2000, size=3).tolist())) for _ in range(steps)]
+    ist = [[(int(i), float(np.float32(rng.uniform(0, 10)))) for i in rng.integers(0, 2000, size=4)] for _ in range(steps)]
+    sim.run(steps, fstim=fst, istim=ist)
+    gc, gi = sim.read_fired()
+    rc, ri = [], []
+    for t in range(steps):
+        f = orc.step(fst[t], ist[t])
+        rc.append(len(f))
+        ri.extend(f)
+    d = first_divergence(np.array(rc), np.array(ri, np.uint32), gc, gi)
+    assert d is None, "first diverging cycle %d: %s" % d
+
+
+def test_if_curr_exp_noise_parity(nemo_mod, oracle_mod):
+    spec = examples.if_curr_exp_80_20(3000, 100, seed=3)
+    sim = _hip_sim(nemo_mod, spec)
+    orc = oracle_mod.from_spec(spec)
+    rc, _ = _compare_run(sim, orc, 800)
+    assert rc.sum() > 0
+    _compare_state(sim, orc, 3000, 4)
+
+
+def test_if_curr_exp_plain_parity(nemo_mod, oracle_mod):
+    spec = examples.if_curr_exp_80_20(2000, 100, seed=4, noise=False)
+    sim = _hip_sim(nemo_mod, spec)
+    orc = oracle_mod.from_spec(spec)
+    rng = np.random.default_rng(1)
+    for t in range(300):
+        ist = [(int(i), 20.0) for i in rng.integers(0, 2000, size=30)]
+        assert sim.step(None, ist) == orc.step(None, ist), t
+    _compare_state(sim, orc, 2000, 4)
+
+
+def test_mixed_types(nemo_mod, oracle_mod):
+    """Izhikevich and IF_curr_exp_noise neurons in one network, interleaved
+    user indices (output in simulation-local order, src/nemo/cpu/Simulation.cpp:199-208)."""
+    a = examples.izhikevich_80_20(1500, 60, seed=21)
+    b = examples.if_curr_exp_80_20(900, 60, seed=22)
+    n = 2400
+    rng = np.random.default_rng(9)
+    perm = rng.permutation(n).astype(np.uint32)
+    groups = [(a.groups[0][0], perm[:1500], a.groups[0][2]), (b.groups[0][0], perm[1500:], b.groups[0][2])]
+    m = 60
+    rowptr = np.arange(0, n * m + 1, m, dtype=np.uint64)
+    targets = rng.integers(0, n, size=n * m, dtype=np.uint32)
+    delays = rng.integers(1, 21, size=n * m, dtype=np.uint8)
+    weights = np.concatenate([a.weights, b.weights]).astype(np.float32)
+    spec = examples.NetSpec(groups, rowptr, targets, delays, weights, np.zeros(n * m, np.uint8))
+    sim = _hip_sim(nemo_mod, spec)
+    orc = oracle_mod.from_spec(spec)
+    _compare_run(sim, orc, 600)
+
+
+def test_stdp_parity(nemo_mod, oracle_mod):
+    """STDP on: excitatory synapses plastic, window A+=0.1, A-=-0.12, tau=20,
+    weights clamped to [0,1], applyStdp(1.0) every 100 cycles: weights and
+    rasters bit-exact."""
+    spec = examples.izhikevich_80_20(2000, 100, seed=13, stdp=True)
+    pre, post = examples.stdp_window()
+    stdp = (pre, post, 0.0, 1.0, 0.0, -1.0)
+    sim = _hip_sim(nemo_mod, spec, stdp=stdp)
+    orc = oracle_mod.from_spec(spec, stdp=stdp)
+    for epoch in range(5):
+        _compare_run(sim, orc, 100)
+        sim.apply_stdp(1.0)
+        orc.apply_stdp(1.0)
+    ref_w = orc.weights_fx(2000, spec.nsyn)
+    sample = np.random.default_rng(0).integers(0, spec.nsyn, size=3000)
+    for k in sample:
+        src = int(np.searchsorted(spec.rowptr, k, side="right") - 1)
+        sid = (src << 32) | int(k - spec.rowptr[src])
+        got = sim.get_synapse_weight(sid)
+        exp = np.float32(np.float32(ref_w[k]) / np.float32(1 << 20))
+        assert got == exp, (k, got, exp)
+    assert np.any(ref_w != np.array([int(np.trunc(np.float64(w) * (1 << 20))) for w in spec.weights], np.int64))
+
+
+def test_synapse_queries(nemo_mod, oracle_mod):
+    spec = examples.izhikevich_80_20(300, 20, seed=2)
+    sim = _hip_sim(nemo_mod, spec)
+    orc = oracle_mod.from_spec(spec)
+    for src in [0, 17, 299]:
+        ids = sim.get_synapses_from(src)
+        assert len(ids) == 20
+        for sid in ids:
+            assert sim.get_synapse_target(sid) == orc.get_synapse_target(sid)
+            assert sim.get_synapse_delay(sid) == orc.get_synapse_delay(sid)
+            assert sim.get_synapse_weight(sid) == orc.get_synapse_weight(sid)
+            assert sim.get_synapse_plastic(sid) == orc.get_synapse_plastic(sid)
+            assert sim.get_synapse_source(sid) == src
+
+
+def test_batch_equals_alone(nemo_mod, oracle_mod):
+    """Networks of uneven size in one batched persistent launch: each raster
+    equals its own oracle run."""
+    sizes = [700, 1500, 3100, 1000, 5000]
+    specs = [examples.izhikevich_80_20(n, 100, seed=100 + i) for i, n in enumerate(sizes)]
+    sims = [_hip_sim(nemo_mod, s) for s in specs]
+    steps = [400, 300, 400, 250, 400]
+    nemo_mod.run_batch(sims, steps)
+    for s, sim, k in zip(specs, sims, steps):
+        orc = oracle_mod.from_spec(s)
+        rc, ri = orc.run(k)
+        gc, gi = sim.read_fired()
+        d = first_divergence(rc, ri, gc, gi)
+        assert d is None, "network %d: first diverging cycle %d: %s" % ((s.n,) + d)
+
+
+@pytest.mark.slow
+def test_c2_full_network_prefix(nemo_mod, oracle_mod):
+    """The 200k x 1000 network (2e8 synapses) for its first 100 cycles: full
+    raster bit-exact against the oracle."""
+    spec = examples.izhikevich_80_20(200000, 1000, seed=1)
+    sim = _hip_sim(nemo_mod, spec)
+    orc = oracle_mod.from_spec(spec)
+    _compare_run(sim, orc, 100)

@@ -1,0 +1,57 @@
+"""NeMo's closed-form step tests on the HIP backend, at several partition
+sizes (the reference's "different partition size => identical raster"
+pattern, src/test/test.cpp:104-132)."""
+import pytest
+
+import scenarios
+from backends import HipBuilder
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture
+def make(nemo_mod):
+    return lambda: HipBuilder(nemo_mod)
+
+
+@pytest.mark.parametrize("ncount,delay,ps", [(1000, 1, None), (1000, 1, 64), (1000, 3, 128), (2000, 20, None)])
+def test_ring(make, ncount, delay, ps):
+    scenarios.ring(make, ncount, delay, partition_size=ps)
+
+
+@pytest.mark.parametrize("ps", [None, 64])
+def test_double_ring(make, ps):
+    scenarios.double_ring(make, partition_size=ps)
+
+
+def test_firing_stimulus(make):
+    scenarios.firing_stimulus(make, cycles=200)
+
+
+def test_current_stimulus(make):
+    scenarios.current_stimulus(make)
+
+
+def test_invalid_current_stimulus(make, nemo_mod):
+    scenarios.invalid_current_stimulus(make, nemo_mod.NemoError)
+
+
+def test_fixed_point_saturation(make, oracle_mod):
+    from backends import OracleBuilder
+    got = scenarios.fixed_point_saturation(make)
+    ref = scenarios.fixed_point_saturation(lambda: OracleBuilder(oracle_mod))
+    assert got == ref
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_integrator_brute_force(make, seed):
+    scenarios.integrator_brute_force(make, seed=seed)
+
+
+def test_integrator_brute_force_saturating(make):
+    scenarios.integrator_brute_force(make, seed=11, saturating=True, nsrc=30)
+
+
+@pytest.mark.parametrize("reward,noise", [(1.0, False), (0.8, False), (0.9, True)])
+def test_stdp_closed_form(make, reward, noise):
+    scenarios.stdp_closed_form(make, reward=reward, noise=noise)

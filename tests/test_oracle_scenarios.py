@@ -1,0 +1,76 @@
+"""NeMo's closed-form step tests, run on the oracle (pins the oracle's step
+semantics: delivery delays, forced firing, current injection, saturation,
+STDP window arithmetic)."""
+import pytest
+
+import scenarios
+from backends import OracleBuilder
+
+
+@pytest.fixture
+def make(oracle_mod):
+    return lambda: OracleBuilder(oracle_mod)
+
+
+@pytest.mark.parametrize("ncount,delay", [(1000, 1), (1000, 3), (2000, 20)])
+def test_ring(make, ncount, delay):
+    scenarios.ring(make, ncount, delay)
+
+
+def test_double_ring(make):
+    scenarios.double_ring(make)
+
+
+def test_firing_stimulus(make):
+    scenarios.firing_stimulus(make, cycles=200)
+
+
+def test_current_stimulus(make):
+    scenarios.current_stimulus(make)
+
+
+def test_invalid_current_stimulus(make, oracle_mod):
+    scenarios.invalid_current_stimulus(make, oracle_mod.OracleError)
+
+
+def test_fixed_point_saturation(make):
+    scenarios.fixed_point_saturation(make)
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_integrator_brute_force(make, seed):
+    scenarios.integrator_brute_force(make, seed=seed)
+
+
+def test_integrator_brute_force_saturating(make):
+    scenarios.integrator_brute_force(make, seed=11, saturating=True, nsrc=30)
+
+
+@pytest.mark.parametrize("reward,noise", [(1.0, False), (0.8, False), (0.9, True)])
+def test_stdp_closed_form(make, reward, noise):
+    scenarios.stdp_closed_form(make, reward=reward, noise=noise)
+
+
+def test_invalid_synapse_target(oracle_mod):
+    o = oracle_mod.Oracle()
+    o.add_neuron("Izhikevich", 0, 0.02, 0.2, -65, 8, 0, -13, -65)
+    o.add_synapse(0, 5, 1, 1.0)
+    with pytest.raises(oracle_mod.OracleError) as e:
+        o.finalize()
+    assert e.value.status == 6
+
+
+def test_duplicate_neuron(oracle_mod):
+    o = oracle_mod.Oracle()
+    o.add_neuron("Izhikevich", 0, 0.02, 0.2, -65, 8, 0, -13, -65)
+    with pytest.raises(oracle_mod.OracleError):
+        o.add_neuron("Izhikevich", 0, 0.02, 0.2, -65, 8, 0, -13, -65)
+
+
+def test_apply_stdp_without_stdp(oracle_mod):
+    o = oracle_mod.Oracle()
+    o.add_neuron("Izhikevich", 0, 0.02, 0.2, -65, 8, 0, -13, -65)
+    o.finalize()
+    with pytest.raises(oracle_mod.OracleError) as e:
+        o.apply_stdp(1.0)
+    assert e.value.status == 9
