@@ -48,6 +48,16 @@ nemo_status_t nemo_set_hip_backend(nemo_configuration_t conf, int dev);
  * Configuration::setCudaPartitionSize (src/nemo/Configuration.hpp:61). */
 nemo_status_t nemo_set_partition_size(nemo_configuration_t conf, unsigned neurons);
 
+/* Which step implementation runs a simulation:
+ *   NEMO_STEP_AUTO    the fused persistent kernel when every neuron type is a
+ *                     built-in model and the network fits it; else plugins
+ *   NEMO_STEP_FUSED   the fused kernel, or fail at nemo_new_simulation
+ *   NEMO_STEP_PLUGINS one kernel sequence per cycle around each type's
+ *                     plugin (lib<Model>.so, include/nemo_hip_plugin.h)
+ * Both give identical results. */
+enum { NEMO_STEP_AUTO = 0, NEMO_STEP_FUSED = 1, NEMO_STEP_PLUGINS = 2 };
+nemo_status_t nemo_set_step_mode(nemo_configuration_t conf, int mode);
+
 /* ------------------------------------------------------- multi-cycle runs */
 
 /*

@@ -33,6 +33,19 @@ enum { IF_V_REST = 0, IF_V_RESET = 1, IF_C_M = 2, IF_TAU_M = 3, IF_TAU_REFRAC = 
 	IF_TAU_SYN_E = 5, IF_TAU_SYN_I = 6, IF_V_THRESH = 7, IF_I_OFFSET = 8, IF_SIGMA = 9 };
 enum { IF_V = 0, IF_IE = 1, IF_II = 2, IF_LASTFIRED = 3 };
 
+/* Parameter / state counts and RNG use per model (the model descriptors,
+ * src/nemo/plugins/Izhikevich.ini:1-15, src/nemo/plugins/IF_curr_exp.ini:14-36) */
+struct ModelInfoDev { int nparam; int nstate; bool normalRng; };
+
+NEMO_HD inline ModelInfoDev model_info_dev(int m)
+{
+	ModelInfoDev r;
+	if(m == MODEL_IZHIKEVICH) { r.nparam = 5; r.nstate = 2; r.normalRng = true; }
+	else if(m == MODEL_IF_CURR_EXP) { r.nparam = 9; r.nstate = 4; r.normalRng = false; }
+	else { r.nparam = 10; r.nstate = 4; r.normalRng = true; }
+	return r;
+}
+
 struct Rng { uint32_t s[4]; };
 
 /* xorshift128, src/nemo/RNG.cpp:9-18 */

@@ -64,6 +64,11 @@ __device__ inline uint32_t lds_prefix_waves(uint32_t* wsum, uint32_t wave, uint3
  * the launch falls through, so all waves always run to completion. */
 __device__ inline void net_barrier(uint32_t* bar, uint32_t target, uint32_t spinLimit, uint32_t* status)
 {
+	/* __syncthreads() waits only for LDS traffic (lgkmcnt); every wave must
+	 * first drain its own outstanding global stores (fired lists, firing
+	 * history) so that they are in L2 when thread 0's agent-scope release
+	 * (buffer_wbl2) writes the L2 back for the other XCDs. */
+	__builtin_amdgcn_s_waitcnt(0);
 	__syncthreads();
 	if(threadIdx.x == 0) {
 		__threadfence();

@@ -31,21 +31,33 @@ std::string fmt(Args&&... args)
 	return s.str();
 }
 
-/* Built-in neuron models. Counts and membrane-potential index follow the
- * model descriptors src/nemo/plugins/Izhikevich.ini:1-5 and
- * src/nemo/plugins/IF_curr_exp.ini:14-26; IF_curr_exp_noise is
- * IF_curr_exp with a 10th parameter sigma (Gaussian input current). */
+/* A neuron model: its descriptor (NeMo's <Model>.ini: parameters=,
+ * state-variables=, membrane-potential=, [rng] normal=,
+ * src/nemo/NeuronType.cpp:92-155) and where its plugin lives. Built-in
+ * models (Izhikevich, IF_curr_exp, IF_curr_exp_noise) also run in the fused
+ * step kernel; any other model found on the plugin path runs through its
+ * plugin's hip_update_neurons. */
 struct ModelInfo {
-	const char* name;
+	std::string name;
 	unsigned id;
 	unsigned nparam;
 	unsigned nstate;
 	unsigned membranePotential;
 	bool normalRng;
+	bool builtin;
+	std::string dir;      /* directory of the descriptor; plugin in dir/hip/ */
 };
 
+/* Resolve a model name: built-ins first, then <dir>/<name>.ini on the plugin
+ * path (most recently added directory first, then the library's own plugin
+ * directory). Throws NEMO_DL_ERROR if not found. */
 const ModelInfo* findModel(const std::string& name);
 const ModelInfo& modelInfo(unsigned id);
+
+/* The plugin search path (src/nemo/Plugin.cpp:84-121): user directories,
+ * most recent first, then <directory of libnemo_hip.so>/plugins. */
+void addPluginPath(const std::string& dir);
+std::vector<std::string> pluginPaths();
 
 /* Q11.20 conversion, src/nemo/fixedpoint.cpp:18-29 (truncating; throws on
  * overflow) */

@@ -29,7 +29,6 @@ struct nemo_simulation_class {
 
 static std::string g_lastError;
 static nemo_status_t g_lastStatus = NEMO_OK;
-static std::vector<std::string> g_pluginPaths;
 
 #define CALL(...) do { \
 	g_lastStatus = NEMO_OK; \
@@ -57,7 +56,7 @@ nemo_add_plugin_path(const char* dir)
 {
 	RETURN_CALL(
 		if(!dir) throw nemo::exception(NEMO_INVALID_INPUT, "null plugin path");
-		g_pluginPaths.insert(g_pluginPaths.begin(), std::string(dir));
+		nemo::addPluginPath(std::string(dir));
 	);
 }
 
@@ -144,6 +143,17 @@ nemo_backend_description(nemo_configuration_t c, const char** descr)
 nemo_status_t nemo_set_write_only_synapses(nemo_configuration_t c) { RETURN_CALL(c->impl.writeOnlySynapses = true); }
 
 nemo_status_t nemo_set_partition_size(nemo_configuration_t c, unsigned n) { RETURN_CALL(c->impl.partitionSize = n); }
+
+nemo_status_t
+nemo_set_step_mode(nemo_configuration_t c, int mode)
+{
+	RETURN_CALL(
+		if(mode != NEMO_STEP_AUTO && mode != NEMO_STEP_FUSED && mode != NEMO_STEP_PLUGINS) {
+			throw nemo::exception(NEMO_INVALID_INPUT, nemo::fmt("Invalid step mode ", mode));
+		}
+		c->impl.stepMode = mode;
+	);
+}
 
 /* ----------------------------------------------------------------- construction */
 
@@ -281,6 +291,18 @@ nemo_hip_run_batch(nemo_simulation_t sims[], size_t n, const unsigned nsteps[], 
 			size_t j = i;
 			while(j < n) {
 				Simulation* sim = sims[j]->sim;
+				if(!sim->fused()) {
+					/* plugin-path simulations run on their own, in order */
+					if(j == i) {
+						unsigned long long ev = 0;
+						sim->run(nsteps ? nsteps[j] : 0, nullptr, &ev);
+						if(events) events[j] = ev;
+						++i;
+						++j;
+						continue;
+					}
+					break;
+				}
 				uint32_t bd = std::max(blockDim, sim->blockThreads());
 				uint32_t ld = std::max(lds, sim->ldsBytes());
 				uint32_t maxB = 0;

@@ -11,9 +11,9 @@ NetworkImpl::NetworkImpl() :
 {
 }
 
-/* src/nemo/NetworkImpl.cpp:38-48: a name maps to one type id. Model names are
- * resolved against the built-in model table (the descriptors NeMo reads from
- * <plugindir>/<name>.ini). */
+/* src/nemo/NetworkImpl.cpp:38-48: a name maps to one type id. Model names
+ * resolve to a built-in model or a descriptor on the plugin path
+ * (src/nemo/NeuronType.cpp:46-70). */
 unsigned
 NetworkImpl::addNeuronType(const std::string& name)
 {
@@ -22,10 +22,6 @@ NetworkImpl::addNeuronType(const std::string& name)
 		return i->second;
 	}
 	const ModelInfo* m = findModel(name);
-	if(!m) {
-		throw exception(NEMO_DL_ERROR, fmt("Unknown neuron model '", name,
-					"' (HIP backend models: Izhikevich, IF_curr_exp, IF_curr_exp_noise)"));
-	}
 	unsigned id = unsigned(groups.size());
 	NeuronGroup g;
 	g.model = m;

@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cstring>
 #include <random>
+#include <dlfcn.h>
 #include <unordered_map>
 
 #include <hip/hip_runtime.h>
@@ -229,6 +230,21 @@ Simulation::Simulation(const NetworkImpl& net, const ConfigurationImpl& conf) :
 		}
 	}
 
+	/* ---- which step runs this network: the fused kernel needs built-in
+	 * models, an LDS ring that fits and all partitions co-resident */
+	bool allBuiltin = true;
+	for(const NeuronGroup& g : net.groups) {
+		if(!g.gidx.empty() && !g.model->builtin) allBuiltin = false;
+		m_np = std::max(m_np, g.model->nparam);
+		m_ns = std::max(m_ns, g.model->nstate);
+	}
+	std::string whyNotFused;
+	if(conf.stepMode == NEMO_STEP_PLUGINS) {
+		whyNotFused = "plugin step requested";
+	} else if(!allBuiltin) {
+		whyNotFused = "a neuron type has no fused implementation";
+	}
+
 	/* T: neurons per partition. Automatic choice aims at one partition per
 	 * compute unit; the LDS ring [D][T] bounds it from above. */
 	uint32_t T;
@@ -245,27 +261,41 @@ Simulation::Simulation(const NetworkImpl& net, const ConfigurationImpl& conf) :
 	while(T > 64 && ldsFor(T) > ldsLimit) {
 		T -= 64;
 	}
-	if(ldsFor(T) > ldsLimit) {
-		throw exception(NEMO_API_UNSUPPORTED, "maximum delay too large for the LDS delay ring");
+	if(whyNotFused.empty() && ldsFor(T) > ldsLimit) {
+		whyNotFused = "maximum delay too large for the LDS delay ring";
 	}
 	m_T = T;
 	m_P = (m_n + T - 1) / T;
-	m_words = m_P * (T / 64u);
 	m_lds = ldsFor(T);
-	if(m_P > 0) {
+	if(whyNotFused.empty() && m_P > 0) {
 		uint32_t maxBlocks = 0;
 		hipCheck(nemo_dev_step_max_blocks(m_T, m_lds, &maxBlocks), "occupancy query");
 		if(m_P > maxBlocks) {
-			throw exception(NEMO_API_UNSUPPORTED, fmt("network of ", m_n, " neurons needs ", m_P,
-						" co-resident workgroups of ", m_T, " neurons; the device holds ", maxBlocks));
+			whyNotFused = fmt("network of ", m_n, " neurons needs ", m_P,
+					" co-resident workgroups of ", m_T, " neurons; the device holds ", maxBlocks);
 		}
+	}
+	if(!whyNotFused.empty()) {
+		if(conf.stepMode == NEMO_STEP_FUSED) {
+			throw exception(NEMO_API_UNSUPPORTED, "fused step kernel unavailable: " + whyNotFused);
+		}
+		/* plugin path: one bucket per source holding all its synapses, with
+		 * absolute targets */
+		m_fused = false;
+		m_P = 1;
+		m_T = 0;
+		m_lds = 0;
+		m_words32 = 2 * ((m_n + 63) / 64);
+		m_words = m_words32 / 2;
+	} else {
+		m_words = m_P * (T / 64u);
 	}
 
 	/* ---- forward matrix, bucketed by (source, target partition) */
 	const uint32_t P1 = m_P + 1;
 	std::vector<uint32_t> off(size_t(m_n) * P1 + 1, 0u);
 	for(size_t i = 0; i < m_nsyn; ++i) {
-		off[size_t(lsrc[i]) * P1 + ltgt[i] / T] += 1;
+		off[size_t(lsrc[i]) * P1 + (m_fused ? ltgt[i] / T : 0u)] += 1;
 	}
 	uint64_t run = 0;
 	for(size_t k = 0; k < off.size(); ++k) {
@@ -280,11 +310,22 @@ Simulation::Simulation(const NetworkImpl& net, const ConfigurationImpl& conf) :
 	std::vector<uint32_t> syn(2 * m_nsyn);
 	std::vector<uint32_t> fwd(m_nsyn);
 	for(size_t i = 0; i < m_nsyn; ++i) {
-		uint32_t lt = ltgt[i], p = lt / T;
+		uint32_t lt = ltgt[i];
+		uint32_t p = m_fused ? lt / T : 0u;
 		uint32_t pos = cursor[size_t(lsrc[i]) * P1 + p]++;
-		syn[2 * size_t(pos)] = (lt - p * T) | (uint32_t(Dl[i]) << 16);
+		if(m_fused) {
+			syn[2 * size_t(pos)] = (lt - p * T) | (uint32_t(Dl[i]) << 16);
+		} else {
+			syn[2 * size_t(pos)] = lt | (uint32_t(Dl[i] - 1) << 26);
+		}
 		syn[2 * size_t(pos) + 1] = uint32_t(wfx[i]);
 		fwd[i] = pos;
+	}
+	if(!m_fused) {
+		/* rows per source: off[s] .. off[s+1] */
+		std::vector<uint32_t> rows(size_t(m_n) + 1, 0u);
+		for(uint32_t s = 0; s <= m_n; ++s) rows[s] = off[size_t(s) * 2];
+		off.swap(rows);
 	}
 	std::vector<uint32_t>().swap(cursor);
 	std::vector<int32_t>().swap(wfx);
@@ -329,8 +370,8 @@ Simulation::Simulation(const NetworkImpl& net, const ConfigurationImpl& conf) :
 	std::vector<uint32_t>().swap(fwd);
 
 	/* ---- neuron arrays */
-	std::vector<float> param(size_t(nemo_dev::MAX_PARAM) * m_n, 0.0f);
-	std::vector<float> state(size_t(nemo_dev::MAX_STATE) * m_n, 0.0f);
+	std::vector<float> param(size_t(m_np) * m_n, 0.0f);
+	std::vector<float> state(size_t(m_ns) * m_n, 0.0f);
 	{
 		uint32_t l = 0;
 		for(const NeuronGroup& g : net.groups) {
@@ -377,23 +418,77 @@ Simulation::Simulation(const NetworkImpl& net, const ConfigurationImpl& conf) :
 	if(d_rcmAcc.bytes) hipCheck(nemo_dev_fill_u32(d_rcmAcc.as<uint32_t>(), 0, d_rcmAcc.bytes / 4, s), "fill");
 	hipCheck(nemo_dev_fill_u32(d_ctl.as<uint32_t>(), 0, 16, s), "fill");
 
-	std::vector<std::pair<uint32_t, uint32_t> > items;
-	for(uint32_t p = 0; p < m_P; ++p) items.push_back(std::make_pair(0u, p));
-	std::vector<WorkItem> work = xcdWorkTable(items);
-	d_work.upload(work.data(), work.size() * sizeof(WorkItem));
-	d_net.alloc(sizeof(NetDev));
+	if(m_fused) {
+		std::vector<std::pair<uint32_t, uint32_t> > items;
+		for(uint32_t p = 0; p < m_P; ++p) items.push_back(std::make_pair(0u, p));
+		std::vector<WorkItem> work = xcdWorkTable(items);
+		d_work.upload(work.data(), work.size() * sizeof(WorkItem));
+		d_net.alloc(sizeof(NetDev));
+	} else {
+		/* pending current as int64 ring [2][D][n], converted inputs, bits */
+		d_acc.alloc(std::max<size_t>(size_t(2) * m_D * m_n, 1) * 8);
+		d_curE.alloc(std::max<size_t>(m_n, 1) * 4);
+		d_curI.alloc(std::max<size_t>(m_n, 1) * 4);
+		d_curX.alloc(std::max<size_t>(m_n, 1) * 4);
+		d_fstim.alloc(std::max<size_t>(m_words32, 2) * 4);
+		d_list.alloc(std::max<size_t>(m_n, 1) * 4 + 4);
+		hipCheck(nemo_dev_fill_u64(d_acc.as<uint64_t>(), 0, d_acc.bytes / 8, s), "fill");
+		loadPlugins(net);
+	}
 	hipCheck(hipStreamSynchronize(s), "hipStreamSynchronize");
 
 	m_deviceBytes = d_model.bytes + d_param.bytes + d_state.bytes + d_rng.bytes + d_recent.bytes
 		+ d_ring.bytes + d_off.bytes + d_syn.bytes + d_fl.bytes + d_fc.bytes + d_rfg.bytes
-		+ d_rcmStart.bytes + d_rcmSd.bytes + d_rcmFwd.bytes + d_rcmAcc.bytes;
+		+ d_rcmStart.bytes + d_rcmSd.bytes + d_rcmFwd.bytes + d_rcmAcc.bytes + d_acc.bytes;
 
 	if(conf.logging) {
-		fprintf(stdout, "nemosim HIP: %u neurons, %zu synapses, %u partitions x %u, max delay %u, "
-				"%s accumulation, %.1f MB device\n", m_n, m_nsyn, m_P, m_T, m_D,
-				m_saturate ? "saturating" : "int32", double(m_deviceBytes) / 1e6);
+		fprintf(stdout, "nemosim HIP: %u neurons, %zu synapses, %s step, %u partitions x %u, max delay %u, "
+				"%s accumulation, %.1f MB device\n", m_n, m_nsyn, m_fused ? "fused" : "plugin",
+				m_P, m_T, m_D, m_saturate ? "saturating" : "int32", double(m_deviceBytes) / 1e6);
 	}
 	resetTimer();
+}
+
+/* Load lib<Model>.so for every neuron type (src/nemo/cuda/Neurons.cpp:43-44,
+ * src/nemo/Plugin.cpp:84-121): the descriptor's own directory first, then
+ * the plugin path; each directory's plugins live in its hip/ subdirectory. */
+void
+Simulation::loadPlugins(const NetworkImpl& net)
+{
+	unsigned base = 0;
+	for(const NeuronGroup& g : net.groups) {
+		unsigned cnt = unsigned(g.gidx.size());
+		if(cnt == 0) continue;
+		std::vector<std::string> dirs;
+		if(!g.model->dir.empty()) dirs.push_back(g.model->dir);
+		for(const std::string& d : pluginPaths()) dirs.push_back(d);
+		void* h = nullptr;
+		std::string tried;
+		for(const std::string& d : dirs) {
+			std::string path = d + "/hip/lib" + g.model->name + ".so";
+			h = dlopen(path.c_str(), RTLD_NOW | RTLD_LOCAL);
+			if(h) break;
+			tried += " " + path;
+		}
+		if(!h) {
+			throw exception(NEMO_DL_ERROR, fmt("No HIP plugin for neuron model ", g.model->name, " (tried", tried, ")"));
+		}
+		PluginGroup pg;
+		pg.start = base;
+		pg.end = base + cnt;
+		pg.handle = h;
+		pg.update = reinterpret_cast<hip_update_neurons_t*>(dlsym(h, "hip_update_neurons"));
+		hip_init_neurons_t* init = reinterpret_cast<hip_init_neurons_t*>(dlsym(h, "hip_init_neurons"));
+		if(!pg.update || !init) {
+			dlclose(h);
+			throw exception(NEMO_DL_ERROR, fmt("Plugin for ", g.model->name, " lacks hip_update_neurons / hip_init_neurons"));
+		}
+		m_plugins.push_back(pg);
+		hipCheck(init(m_stream, pg.start, pg.end, d_param.as<float>() + pg.start, m_n,
+					d_state.as<float>() + pg.start, m_n, d_rng.as<uint32_t>() + pg.start, m_n),
+				"hip_init_neurons");
+		base += cnt;
+	}
 }
 
 Simulation::~Simulation()
@@ -401,6 +496,9 @@ Simulation::~Simulation()
 	if(m_stream) {
 		(void) hipStreamSynchronize(static_cast<hipStream_t>(m_stream));
 		(void) hipStreamDestroy(static_cast<hipStream_t>(m_stream));
+	}
+	for(PluginGroup& pg : m_plugins) {
+		if(pg.handle) dlclose(pg.handle);
 	}
 }
 
@@ -470,6 +568,8 @@ Simulation::uploadSchedule(unsigned nsteps, const nemo_schedule_t* s)
 	}
 	/* the host vectors go out of scope: wait for the copies */
 	hipCheck(hipStreamSynchronize(st), "hipStreamSynchronize");
+	m_fsPtrHost.swap(fsPtr);
+	m_isPtrHost.swap(isPtr);
 	m_hasSchedule = true;
 }
 
@@ -542,11 +642,98 @@ Simulation::finishLaunch(unsigned nsteps, void* stream, unsigned long long* even
 	}
 }
 
+/* The modular step: per cycle, convert the due ring slot, apply stimulus,
+ * run each type's plugin, compact the fired bits, scatter the fired
+ * sources' synapses into the int64 ring, accumulate STDP
+ * (src/nemo/cuda/Simulation.cpp:336-430 has the same phase structure). */
+void
+Simulation::launchPlugins(unsigned nsteps, unsigned long long* events)
+{
+	hipStream_t st = static_cast<hipStream_t>(m_stream);
+	hipCheck(hipSetDevice(m_device), "hipSetDevice");
+	d_raster.ensure(std::max<size_t>(size_t(nsteps) * m_words32, 2) * 4);
+	hipCheck(nemo_dev_fill_u32(d_ctl.as<uint32_t>(), 0, 16, st), "fill");
+	PluginStepDev s;
+	std::memset(&s, 0, sizeof(s));
+	s.n = m_n;
+	s.D = m_D;
+	s.words32 = m_words32;
+	s.cycle0 = m_cycle;
+	s.acc = d_acc.as<long long>();
+	s.curE = d_curE.as<float>();
+	s.curI = d_curI.as<float>();
+	s.curX = d_curX.as<float>();
+	s.fstim = d_fstim.as<uint32_t>();
+	s.raster = d_raster.as<uint32_t>();
+	s.list = d_list.as<uint32_t>();
+	s.nfired = d_list.as<uint32_t>() + m_n;
+	s.off = d_off.as<uint32_t>();
+	s.syn = d_syn.as<uint32_t>();
+	s.recent = d_recent.as<uint64_t>();
+	s.events = reinterpret_cast<unsigned long long*>(d_ctl.as<uint32_t>() + 2);
+	s.rcm_start = d_rcmStart.as<uint32_t>();
+	s.rcm_sd = d_rcmSd.as<uint32_t>();
+	s.rcm_acc = d_rcmAcc.as<int32_t>();
+	s.stdp = d_stdp.ptr;
+	if(m_hasSchedule) {
+		s.fs_ptr = d_fsPtr.as<uint32_t>();
+		s.fs_idx = d_fsIdx.as<uint32_t>();
+		s.is_ptr = d_isPtr.as<uint32_t>();
+		s.is_idx = d_isIdx.as<uint32_t>();
+		s.is_val = d_isVal.as<float>();
+	}
+	hipEvent_t e0, e1;
+	hipCheck(hipEventCreate(&e0), "hipEventCreate");
+	hipCheck(hipEventCreate(&e1), "hipEventCreate");
+	hipCheck(hipEventRecord(e0, st), "hipEventRecord");
+	for(unsigned k = 0; k < nsteps; ++k) {
+		hipCheck(nemo_dev_pm_begin(&s, k, st), "plugin step: begin");
+		if(m_hasSchedule) {
+			uint32_t entries = (m_fsPtrHost[k + 1] - m_fsPtrHost[k]) + (m_isPtrHost[k + 1] - m_isPtrHost[k]);
+			hipCheck(nemo_dev_pm_stimulus(&s, k, entries, st), "plugin step: stimulus");
+		}
+		uint32_t* firedRow = s.raster + size_t(k) * m_words32;
+		for(const PluginGroup& pg : m_plugins) {
+			hipCheck(pg.update(m_stream, unsigned(m_cycle + k), pg.start, pg.end,
+						d_param.as<float>() + pg.start, m_n, d_state.as<float>() + pg.start, m_n,
+						d_rng.as<uint32_t>() + pg.start, m_n, s.fstim, s.curE, s.curI, s.curX,
+						s.recent, firedRow), "hip_update_neurons");
+		}
+		hipCheck(nemo_dev_pm_compact(&s, k, st), "plugin step: compact");
+		hipCheck(nemo_dev_pm_scatter(&s, k, st), "plugin step: scatter");
+		if(m_stdp) {
+			hipCheck(nemo_dev_pm_stdp(&s, st), "plugin step: stdp");
+		}
+	}
+	hipCheck(hipEventRecord(e1, st), "hipEventRecord");
+	uint32_t ctl[4];
+	hipCheck(hipMemcpyAsync(ctl, d_ctl.ptr, sizeof(ctl), hipMemcpyDeviceToHost, st), "D2H");
+	hipCheck(hipStreamSynchronize(st), "hipStreamSynchronize");
+	float ms = 0.0f;
+	hipCheck(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+	m_kernelMs = ms;
+	(void) hipEventDestroy(e0);
+	(void) hipEventDestroy(e1);
+	m_cycle += nsteps;
+	m_timerCycles += nsteps;
+	m_lastRunSteps = nsteps;
+	m_hasSchedule = false;
+	if(events) {
+		unsigned long long e;
+		std::memcpy(&e, &ctl[2], sizeof(e));
+		*events = e;
+	}
+}
+
 void
 Simulation::launch(unsigned nsteps, unsigned long long* events)
 {
 	if(events) *events = 0;
-	if(m_P == 0 || nsteps == 0) {
+	if(!m_fused && m_n > 0 && nsteps > 0) {
+		launchPlugins(nsteps, events);
+		return;
+	}
+	if(m_P == 0 || m_n == 0 || nsteps == 0) {
 		m_cycle += nsteps;
 		m_timerCycles += nsteps;
 		m_lastRunSteps = nsteps;

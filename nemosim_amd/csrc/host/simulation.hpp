@@ -10,7 +10,9 @@
 #include "network.hpp"
 #include "../device/models.h"
 #include "../device/step_abi.h"
+#include "../device/plugin_abi.h"
 #include "nemo_hip.h"
+#include "nemo_hip_plugin.h"
 
 namespace nemo {
 namespace hip {
@@ -25,6 +27,7 @@ struct ConfigurationImpl {
 	bool writeOnlySynapses = false;
 	bool logging = false;
 	unsigned partitionSize = 0;   /* 0 = automatic */
+	int stepMode = NEMO_STEP_AUTO;
 	std::string description;
 
 	/* src/nemo/StdpFunction.cpp:12-50 */
@@ -97,11 +100,28 @@ class Simulation
 		NetDev prepareLaunch(unsigned nsteps, void* stream);
 		void finishLaunch(unsigned nsteps, void* stream, unsigned long long* events);
 		unsigned neuronCount() const { return m_n; }
+		/* true: the fused persistent kernel; false: plugin kernels per cycle */
+		bool fused() const { return m_fused; }
 
 		static std::vector<WorkItem> xcdWorkTable(const std::vector<std::pair<uint32_t, uint32_t> >& items);
 
 	private:
+		bool m_fused = true;
 		uint32_t m_n = 0, m_P = 0, m_T = 64, m_D = 1, m_words = 0, m_lds = 0;
+		uint32_t m_np = nemo_dev::MAX_PARAM, m_ns = nemo_dev::MAX_STATE;
+
+		/* plugin path */
+		struct PluginGroup {
+			unsigned start, end;
+			void* handle;
+			hip_update_neurons_t* update;
+		};
+		std::vector<PluginGroup> m_plugins;
+		uint32_t m_words32 = 0;
+		DevBuf d_acc, d_curE, d_curI, d_curX, d_fstim, d_list;
+		std::vector<uint32_t> m_fsPtrHost, m_isPtrHost;
+		void loadPlugins(const NetworkImpl& net);
+		void launchPlugins(unsigned nsteps, unsigned long long* events);
 		bool m_saturate = false;
 		bool m_stdp = false;
 		bool m_writeOnly = false;

@@ -18,6 +18,13 @@ NEMO_BUFFER_UNDERFLOW = 8
 NEMO_LOGIC_ERROR = 9
 NEMO_DL_ERROR = 12
 
+STEP_MODES = {"auto": 0, "fused": 1, "plugins": 2}
+
+
+def add_plugin_path(path):
+    """Prepend a directory to the neuron-model plugin path (nemo_add_plugin_path)."""
+    _check(_lib_checked().nemo_add_plugin_path(path.encode()))
+
 
 class NemoError(RuntimeError):
     def __init__(self, status, message):
@@ -84,6 +91,10 @@ class Configuration(object):
 
     def set_partition_size(self, neurons):
         _check(self._lib.nemo_set_partition_size(self._h, int(neurons)))
+
+    def set_step_mode(self, mode):
+        """'auto', 'fused' or 'plugins' (see nemo_set_step_mode)."""
+        _check(self._lib.nemo_set_step_mode(self._h, STEP_MODES[mode]))
 
     def enable_logging(self):
         _check(self._lib.nemo_log_stdout(self._h))

@@ -172,14 +172,22 @@ class Oracle(object):
         _check(self.lib.oracle_step(self.h, _u(fs), len(fs), _u(ii), _f(iv), len(ii), _u(out), ctypes.byref(cnt)))
         return out[:cnt.value].tolist()
 
-    def run(self, nsteps, cap=None):
+    def run(self, nsteps, chunk=100):
+        """nsteps cycles without stimulus: (fired count per cycle, fired ids)."""
         counts = np.zeros(nsteps, np.uint32)
-        if cap is None:
-            cap = max(1, self.n * nsteps // 8 + 1024)
-        ids = np.zeros(cap, np.uint32)
-        total = ctypes.c_size_t()
-        _check(self.lib.oracle_run(self.h, nsteps, _u(counts), _u(ids), cap, ctypes.byref(total)))
-        return counts, ids[:total.value]
+        parts = []
+        done = 0
+        while done < nsteps:
+            k = min(chunk, nsteps - done)
+            cap = max(1, self.n * k)
+            ids = np.zeros(cap, np.uint32)
+            total = ctypes.c_size_t()
+            c = np.zeros(k, np.uint32)
+            _check(self.lib.oracle_run(self.h, k, _u(c), _u(ids), cap, ctypes.byref(total)))
+            counts[done:done + k] = c
+            parts.append(ids[:total.value].copy())
+            done += k
+        return counts, (np.concatenate(parts) if parts else np.zeros(0, np.uint32))
 
     def apply_stdp(self, reward):
         _check(self.lib.oracle_apply_stdp(self.h, reward))

@@ -64,8 +64,10 @@ class HipBuilder(object):
 
 def first_divergence(ref_counts, ref_ids, got_counts, got_ids):
     """(cycle, detail) of the first cycle whose fired set differs, or None."""
-    ro = np.concatenate([[0], np.cumsum(ref_counts)])
-    go = np.concatenate([[0], np.cumsum(got_counts)])
+    ro = np.concatenate([[0], np.cumsum(np.asarray(ref_counts, np.int64))]).astype(np.int64)
+    go = np.concatenate([[0], np.cumsum(np.asarray(got_counts, np.int64))]).astype(np.int64)
+    ref_ids = np.asarray(ref_ids, np.int64)
+    got_ids = np.asarray(got_ids, np.int64)
     for t in range(len(ref_counts)):
         a = ref_ids[ro[t]:ro[t + 1]]
         b = got_ids[go[t]:go[t + 1]]
