@@ -1,20 +1,25 @@
 #!/usr/bin/env python3
 """Benchmark of the HIP simulation step (one GPU).
 
-Workload (default): the Izhikevich 80/20 network at 200,000 neurons x 1000
-synapses (2e8 synapses), delays U{1..20} ms, STDP off, synthetic seeded
-construction, run with the plain single-network persistent step kernel.
-A "step" is one 1 ms simulation cycle of the whole network.
+Default workload ("plain"): the Izhikevich 80/20 network at 200,000 neurons
+x 1000 synapses (2e8 synapses), delays U{1..20} ms, STDP off, synthetic
+seeded construction, run with the plain single-network persistent step
+kernel. A "step" is one 1 ms simulation cycle of the whole network.
 
 Metric: synaptic events per wall second (exact delivered synapses), plus
 ms of wall time per simulated second. Timing follows NeMo's benchmark()
 (src/examples/common.cpp:13-122): warm-up cycles first, then the timed
 cycles; the network and its state are resident on the GPU before timing.
 
-Prints one JSON line with the roofline of the step kernel (algorithmic
-bytes per launch / kernel time from HIP events on the kernel's stream) and a
-CPU baseline: the oracle (a restatement of NeMo's OpenMP CPU backend) timed
-on a bounded sample of the same workload on this host.
+The default prints one JSON line with the roofline of the step kernel
+(algorithmic bytes per launch / kernel time from HIP events on the kernel's
+stream) and a CPU baseline: the oracle (a restatement of NeMo's OpenMP CPU
+backend) timed on a bounded sample of the same workload on this host.
+
+Other workloads (--workload): "stdp" (50k x 1000, STDP, applyStdp every
+1000 cycles), "batch" (256 networks of 1k-20k neurons, batched persistent
+launch vs one plain run per network), "serve" (Poisson requests under an
+HBM budget, p50/p99 latency).
 """
 import argparse
 import json
@@ -54,16 +59,15 @@ def algorithmic_bytes(fired_ids, deg, rows, n_neurons, steps):
     return 8 * e + 8 * r + 68 * n_neurons * steps, e, r
 
 
-def cpu_baseline(spec, target_s, warm_steps):
+def cpu_baseline(spec, target_s, warm_steps, stdp=None):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     log("cpu baseline: building the oracle network ...")
-    orc = oracle.from_spec(spec)
+    orc = oracle.from_spec(spec, stdp=stdp)
     orc.run(warm_steps)
     deg, _ = delay_tables(spec)
-    # probe the step time, then size the sample to ~target_s of CPU work
     t0 = time.perf_counter()
-    c, ids = orc.run(10)
+    orc.run(10)
     probe = (time.perf_counter() - t0) / 10
     nsamp = int(max(20, min(20000, target_s / max(probe, 1e-6))))
     log("cpu baseline: %.2f ms/cycle probe; sampling %d cycles" % (probe * 1e3, nsamp))
@@ -83,63 +87,39 @@ def cpu_baseline(spec, target_s, warm_steps):
     }
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5000, help="timed cycles")
-    ap.add_argument("--warmup", type=int, default=1000, help="warm-up cycles")
-    ap.add_argument("--launch-steps", type=int, default=1000, help="cycles per persistent launch")
-    ap.add_argument("--neurons", type=int, default=200000)
-    ap.add_argument("--synapses", type=int, default=1000)
-    ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    args = ap.parse_args()
-
-    if args.gpus != 1:
-        print("bench.py: this build runs on one GPU; --gpus %d is not supported "
-              "(one network per GPU, see DESIGN.md for how it would shard)" % args.gpus, file=sys.stderr)
-        sys.exit(2)
-
-    import nemosim_amd
-    from nemosim_amd import examples
-
-    n, m = args.neurons, args.synapses
-    log("generating Izhikevich 80/20 network: %d neurons x %d synapses ..." % (n, m))
-    t0 = time.perf_counter()
-    spec = examples.izhikevich_80_20(n, m, seed=args.seed)
-    log("  %.1f s" % (time.perf_counter() - t0))
-    t0 = time.perf_counter()
+def build_sim(nemosim_amd, spec, stdp=None, partition_size=None, mode="auto"):
     net = spec.to_nemo(nemosim_amd)
     conf = nemosim_amd.Configuration()
     conf.set_cuda_backend(0)
     conf.set_write_only_synapses()
-    sim = nemosim_amd.Simulation(net, conf)
-    del net
-    info = sim.info()
-    log("  simulation built in %.1f s: %s" % (time.perf_counter() - t0, info))
+    conf.set_step_mode(mode)
+    if partition_size:
+        conf.set_partition_size(partition_size)
+    if stdp is not None:
+        pre, post, a, b, c, d = stdp
+        conf.set_stdp_function(pre, post, min_e=a, max_e=b, min_i=c, max_i=d)
+    return nemosim_amd.Simulation(net, conf)
 
-    L = max(1, min(args.launch_steps, args.steps))
-    deg, rows = delay_tables(spec)
 
-    log("warm-up: %d cycles" % args.warmup)
+def timed_runs(sim, spec, steps, warmup, L, stdp_period=None):
+    """warm-up, then `steps` timed cycles in launches of L; returns
+    (wall_s, kernel_ms, events, fired ids, launches)."""
     done = 0
-    while done < args.warmup:
-        k = min(L, args.warmup - done)
+    while done < warmup:
+        k = min(L, warmup - done)
         sim.run(k)
         done += k
-
-    log("timed: %d cycles in launches of %d" % (args.steps, L))
-    events = 0
-    kernel_ms = 0.0
+        if stdp_period and done % stdp_period == 0:
+            sim.apply_stdp(1.0)
+    events, kernel_ms, wall, launches = 0, 0.0, 0.0, 0
     fired = []
-    launches = 0
-    wall = 0.0
     done = 0
-    while done < args.steps:
-        k = min(L, args.steps - done)
+    while done < steps:
+        k = min(L, steps - done)
         t0 = time.perf_counter()
         events += sim.run(k)
+        if stdp_period and (warmup + done + k) % stdp_period == 0:
+            sim.apply_stdp(1.0)
         wall += time.perf_counter() - t0
         kernel_ms += sim.info()["last_kernel_ms"]
         launches += 1
@@ -147,12 +127,40 @@ def main():
         fired.append(ids)
         done += k
     fired = np.concatenate(fired) if fired else np.zeros(0, np.uint32)
+    return wall, kernel_ms, events, fired, launches
+
+
+def bench_plain(args, stdp=False):
+    import nemosim_amd
+    from nemosim_amd import examples
+
+    n, m = args.neurons, args.synapses
+    if stdp and n == 200000:
+        n = 50000
+    log("generating Izhikevich 80/20 network: %d neurons x %d synapses%s ..." % (n, m, " (STDP)" if stdp else ""))
+    spec = examples.izhikevich_80_20(n, m, seed=args.seed, stdp=stdp)
+    stdp_fn = None
+    if stdp:
+        pre, post = examples.stdp_window()
+        stdp_fn = (pre, post, 0.0, 1.0, 0.0, -1.0)
+    t0 = time.perf_counter()
+    sim = build_sim(nemosim_amd, spec, stdp=stdp_fn)
+    info = sim.info()
+    log("  simulation built in %.1f s: %s" % (time.perf_counter() - t0, info))
+    L = max(1, min(args.launch_steps, args.steps))
+    deg, rows = delay_tables(spec)
+    log("warm-up %d cycles, timed %d cycles in launches of %d" % (args.warmup, args.steps, L))
+    wall, kernel_ms, events, fired, launches = timed_runs(sim, spec, args.steps, args.warmup, L,
+                                                          stdp_period=1000 if stdp else None)
     abytes, e_check, r_total = algorithmic_bytes(fired.astype(np.int64), deg, rows, n, args.steps)
     if e_check != events:
         log("warning: device event count %d != raster-derived %d" % (events, e_check))
-    kernel_s = kernel_ms / 1e3
-    achieved = abytes / launches / (kernel_s / launches) / 1e9
-
+    if stdp:
+        # plastic targets that fired (postLen cycles later) walk ~0.8 m in-synapses:
+        # 4 B reverse record + 8 B source history + 8 B accumulator read/write each
+        indeg_plastic = 0.8 * m
+        abytes += int(len(fired) * indeg_plastic * 20) + 8 * n * args.steps
+    achieved = abytes / (kernel_ms / 1e3) / 1e9
     result = {
         "metric": "synaptic events/s",
         "value": events / wall,
@@ -168,7 +176,7 @@ def main():
         "dtype": "q11.20-i32/f32/f64",
         "data": "synthetic (seeded Izhikevich 80/20 construction)",
         "config": {
-            "workload": "izhikevich_80_20 plain step, STDP off",
+            "workload": "izhikevich_80_20 plain step, STDP %s" % ("on, applyStdp(1.0) every 1000 cycles" if stdp else "off"),
             "neurons": n,
             "synapses_per_neuron": m,
             "delays": "U{1..20} ms",
@@ -185,14 +193,162 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic_from_profile(args.traffic_file),
             "algorithmic_bytes_per_launch": abytes / launches,
             "kernel_ms_per_launch": kernel_ms / launches,
         },
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:
-        result["cpu_baseline"] = cpu_baseline(spec, args.cpu_seconds, warm_steps=200)
+        result["cpu_baseline"] = cpu_baseline(spec, args.cpu_seconds, warm_steps=200, stdp=stdp_fn)
+    return result
+
+
+def traffic_from_profile(path):
+    """HBM bytes per step-kernel launch from the committed rocprof counter
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration),
+    or None if there is none."""
+    if not path or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def bench_batch(args):
+    """256 networks, N log-uniform in [1k, 20k], 100 synapses, 2000 cycles each:
+    one batched persistent launch set vs one plain run per network."""
+    import nemosim_amd
+    from nemosim_amd import examples, autotune
+    count, steps = args.batch_networks, args.batch_steps
+    sizes = examples.log_uniform_sizes(count, 1000, 20000, seed=args.seed)
+    table = autotune.load_table()
+    log("batch: %d networks, %d neurons total, autotune table %s" % (count, int(sizes.sum()), table))
+    specs = [examples.izhikevich_80_20(int(n), 100, seed=args.seed + 1 + i) for i, n in enumerate(sizes)]
+    t0 = time.perf_counter()
+    plain = [build_sim(nemosim_amd, s) for s in specs]
+    batched = [build_sim(nemosim_amd, s, partition_size=autotune.partition_size(s.n, table)) for s in specs]
+    log("  built %d simulations in %.1f s" % (2 * count, time.perf_counter() - t0))
+    # plain: one persistent run per network, back to back
+    t0 = time.perf_counter()
+    ev_plain = 0
+    for s in plain:
+        ev_plain += s.run(steps)
+    wall_plain = time.perf_counter() - t0
+    # batched: all networks, packed into as few persistent launches as fit
+    t0 = time.perf_counter()
+    ev_batch = int(np.sum(nemosim_amd.run_batch(batched, [steps] * count)))
+    wall_batch = time.perf_counter() - t0
+    exact = 0
+    for a, b in zip(plain, batched):
+        ca, ia = a.read_fired()
+        cb, ib = b.read_fired()
+        exact += int(np.array_equal(ca, cb) and np.array_equal(ia, ib))
+    return {
+        "metric": "synaptic events/s (batched vs plain)",
+        "value": ev_batch / wall_batch,
+        "unit": "synaptic events/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": 0,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "q11.20-i32/f32/f64",
+        "data": "synthetic (seeded Izhikevich 80/20 construction)",
+        "config": {"workload": "%d izhikevich_80_20 networks, N log-uniform [1k,20k], m=100" % count,
+                   "neurons_total": int(sizes.sum())},
+        "plain_events_per_s": ev_plain / wall_plain,
+        "batched_over_plain": (ev_batch / wall_batch) / (ev_plain / wall_plain),
+        "wall_plain_s": wall_plain,
+        "wall_batched_s": wall_batch,
+        "rasters_identical": "%d/%d" % (exact, count),
+    }
+
+
+def bench_serve(args):
+    import nemosim_amd  # noqa: F401
+    from nemosim_amd import serving
+    reqs = serving.poisson_requests(args.serve_requests, rate_hz=50.0, seed=args.seed)
+    budget = int(args.serve_budget_gb * (1 << 30))
+    srv = serving.Server(hbm_budget_bytes=budget, chunk_cycles=args.serve_chunk,
+                         build_workers=args.serve_workers, keep_rasters=False)
+    log("serve: %d requests, budget %.0f GiB, chunks of %d cycles" % (len(reqs), args.serve_budget_gb, args.serve_chunk))
+    stats = srv.serve(reqs, realtime=True)
+    return {
+        "metric": "synaptic events/s (serving)",
+        "value": stats["events_per_s"],
+        "unit": "synaptic events/s",
+        "n_gpus": 1,
+        "steps": stats["sim_cycles"],
+        "warmup": 0,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "q11.20-i32/f32/f64",
+        "data": "synthetic (Poisson 50 req/s, N log-uniform [1k,50k], m U{100..1000}, 30% STDP, 30% IF_curr_exp_noise)",
+        "config": {"workload": "serving", "requests": stats["requests"], "budget_gib": args.serve_budget_gb,
+                   "chunk_cycles": args.serve_chunk},
+        "p50_latency_s": stats["p50_latency_s"],
+        "p99_latency_s": stats["p99_latency_s"],
+        "requests_per_s": stats["requests_per_s"],
+        "gpu_busy_s": stats["gpu_busy_s"],
+        "wall_s": stats["wall_s"],
+    }
+
+
+def copy_calibration():
+    """A plain device copy of known size (1 GiB read + 1 GiB written per
+    launch, 4 launches) for scaling rocprof's FETCH_SIZE / WRITE_SIZE to
+    bytes: run under the same counter as the step kernel."""
+    import torch
+    n = 1 << 28                                   # 2^28 float32 = 1 GiB
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(4):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    return {"metric": "copy calibration", "bytes_read_per_launch": 4 * n, "bytes_written_per_launch": 4 * n,
+            "launches": 4, "kernel": "torch copy_ (elementwise copy kernel)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", default="plain", choices=["plain", "stdp", "batch", "serve", "copy-calibration"])
+    ap.add_argument("--steps", type=int, default=5000, help="timed cycles")
+    ap.add_argument("--warmup", type=int, default=1000, help="warm-up cycles")
+    ap.add_argument("--launch-steps", type=int, default=1000, help="cycles per persistent launch")
+    ap.add_argument("--neurons", type=int, default=200000)
+    ap.add_argument("--synapses", type=int, default=1000)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "nemo_step_kernel_c2_counters.txt"))
+    ap.add_argument("--batch-networks", type=int, default=256)
+    ap.add_argument("--batch-steps", type=int, default=2000)
+    ap.add_argument("--serve-requests", type=int, default=200)
+    ap.add_argument("--serve-budget-gb", type=float, default=64.0)
+    ap.add_argument("--serve-chunk", type=int, default=100)
+    ap.add_argument("--serve-workers", type=int, default=8)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; --gpus %d is not supported "
+              "(one network per GPU, see DESIGN.md for how it would shard)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    if args.workload == "plain":
+        result = bench_plain(args)
+    elif args.workload == "stdp":
+        result = bench_plain(args, stdp=True)
+    elif args.workload == "batch":
+        result = bench_batch(args)
+    elif args.workload == "copy-calibration":
+        result = copy_calibration()
+    else:
+        result = bench_serve(args)
     print(json.dumps(result), flush=True)
 
 

@@ -8,7 +8,7 @@
  * cuda_init_neurons_t, src/nemo/cuda/plugins/neuron_model.h:26-62) and CPU
  * plugin ABI (src/nemo/cpu/plugins/neuron_model.h:25-52). A model is a shared
  * library lib<Model>.so in <plugindir>/hip/, described by <plugindir>/<Model>.ini
- * (the same descriptor format as src/nemo/plugins/*.ini: parameters=,
+ * (the same descriptor format as src/nemo/plugins/<Model>.ini: parameters=,
  * state-variables=, membrane-potential=, [rng] normal=). The backend loads it
  * with dlopen and resolves the two symbols below.
  *

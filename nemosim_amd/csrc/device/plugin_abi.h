@@ -49,6 +49,7 @@ int nemo_dev_pm_stimulus(const PluginStepDev* s, uint32_t k, uint32_t entries, v
 int nemo_dev_pm_compact(const PluginStepDev* s, uint32_t k, void* stream);
 int nemo_dev_pm_scatter(const PluginStepDev* s, uint32_t k, void* stream);
 int nemo_dev_pm_stdp(const PluginStepDev* s, void* stream);
+int nemo_dev_pm_replay(const PluginStepDev* s, void* stream);
 
 #ifdef __cplusplus
 }

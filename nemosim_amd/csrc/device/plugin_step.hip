@@ -148,6 +148,40 @@ __global__ void pm_stdp_kernel(PluginStepDev s)
 	}
 }
 
+/* After weights change (applyStdp): rebuild the ring from the firing
+ * history, since NeMo reads a weight when the spike arrives
+ * (src/nemo/cpu/Simulation.cpp:258-307). The ring has been zeroed; a spike
+ * fired at cycle0-1-k with delay d > k is still pending and lands in slot
+ * (cycle0-1-k+d) mod D. One wave per source with pending spikes. */
+__global__ void pm_replay_kernel(PluginStepDev s)
+{
+	const uint32_t lane = threadIdx.x & 63u;
+	const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+	const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+	const uint64_t pend = (s.D >= 64u) ? ~0ull : ((1ull << s.D) - 1ull);
+	const uint32_t last = uint32_t((s.cycle0 + s.D - 1u) % s.D);
+	for(uint32_t src = wave; src < s.n; src += nwaves) {
+		const uint64_t h = s.recent[src] & pend;
+		if(!h) continue;
+		const uint32_t a = s.off[src], b = s.off[src + 1];
+		for(uint32_t q = a + lane; q < b; q += 64u) {
+			const uint32_t x = s.syn[2 * size_t(q)];
+			const int32_t w = int32_t(s.syn[2 * size_t(q) + 1]);
+			const uint32_t tgt = x & 0x3ffffffu;
+			const uint32_t d = (x >> 26) + 1u;
+			const uint32_t plane = (w >= 0) ? 0u : s.D;
+			uint64_t hb = h & ((d >= 64u) ? ~0ull : ((1ull << d) - 1ull));
+			while(hb) {
+				const uint32_t kk = uint32_t(__builtin_ctzll(hb));
+				hb &= hb - 1ull;
+				const uint32_t sl = (last + d - kk) % s.D;
+				atomicAdd(reinterpret_cast<unsigned long long*>(&s.acc[size_t(plane + sl) * s.n + tgt]),
+						(unsigned long long)(long long) w);
+			}
+		}
+	}
+}
+
 unsigned grid(size_t n, unsigned block, unsigned cap)
 {
 	size_t g = (n + block - 1) / block;
@@ -186,6 +220,13 @@ int nemo_dev_pm_scatter(const PluginStepDev* s, uint32_t k, void* stream)
 {
 	hipLaunchKernelGGL(pm_scatter_kernel, dim3(2048), dim3(256), 0,
 			static_cast<hipStream_t>(stream), *s, k);
+	return int(hipGetLastError());
+}
+
+int nemo_dev_pm_replay(const PluginStepDev* s, void* stream)
+{
+	hipLaunchKernelGGL(pm_replay_kernel, dim3(2048), dim3(256), 0,
+			static_cast<hipStream_t>(stream), *s);
 	return int(hipGetLastError());
 }
 

@@ -16,6 +16,8 @@ extern "C" {
 /* Bits of NetDev.flags */
 #define NEMO_NET_STDP      1u   /* accumulate STDP this launch */
 #define NEMO_NET_SATURATE  2u   /* accumulators may leave the int32 range */
+#define NEMO_NET_REPLAY    4u   /* weights changed since the ring was filled:
+                                 * rebuild it from the firing history */
 
 /* Bits of the per-network status word */
 #define NEMO_STATUS_BARRIER_TIMEOUT 1u
@@ -49,11 +51,11 @@ typedef struct NetDev {
 	const uint32_t* off;
 	uint32_t* syn;            /* [2 * synapses] */
 
-	/* per-step exchange between the network's workgroups */
+	/* per-step exchange between the network's workgroups, double-buffered
+	 * by cycle parity */
 	uint32_t* fl;             /* [2][P][T] fired neurons of each partition */
-	uint32_t* fc;             /* [2][P] their counts */
+	uint64_t* fc;             /* [2][P] (cycle << 32) | fired count; starts all-ones */
 	uint64_t* rfg;            /* [2][n] firing history (STDP only) */
-	uint32_t* bar;            /* barrier arrival counter, zeroed before a launch */
 	uint32_t* status;         /* NEMO_STATUS_* */
 	unsigned long long* events; /* delivered-synapse counter */
 

@@ -22,7 +22,9 @@
  *   2. fired-list compaction: wavefront ballot + prefix over waves, giving
  *      the partition's fired neurons in ascending order, and the 64-bit
  *      ballot words as the step's firing raster;
- *   3. barrier of the network's workgroups (the only grid-level sync);
+ *   3. publish the partition's fired count tagged with the cycle, then wait
+ *      for all partitions' counts of the cycle (the only grid-level sync;
+ *      no counter atomics, no L2 write-back or invalidate);
  *   4. delivery: every workgroup walks the fired lists of all partitions and,
  *      for each fired source, reads only its own bucket of the source's
  *      synapses (all delays at once) and adds each weight into ring slot
@@ -58,24 +60,48 @@ __device__ inline uint32_t lds_prefix_waves(uint32_t* wsum, uint32_t wave, uint3
 	return before;
 }
 
-/* Barrier of the workgroups of one network. bar counts arrivals since the
- * launch started; step k waits for (k+1)*P of them. The wait is bounded:
- * after spinLimit polls the status word is flagged and every later barrier of
- * the launch falls through, so all waves always run to completion. */
-__device__ inline void net_barrier(uint32_t* bar, uint32_t target, uint32_t spinLimit, uint32_t* status)
+/* Cross-workgroup data (fired lists, their tagged counts, the STDP firing
+ * history) is exchanged with agent-scope relaxed atomic loads and stores.
+ * On gfx950 these are sc1 accesses, coherent across the XCDs' L2s, so no L2
+ * write-back or invalidate is needed. A writer drains its stores
+ * (s_waitcnt; __syncthreads alone waits only for LDS traffic) before
+ * publishing the count that names them. */
+template<typename V>
+__device__ inline void st_agent(V* p, V v)
 {
-	/* __syncthreads() waits only for LDS traffic (lgkmcnt); every wave must
-	 * first drain its own outstanding global stores (fired lists, firing
-	 * history) so that they are in L2 when thread 0's agent-scope release
-	 * (buffer_wbl2) writes the L2 back for the other XCDs. */
-	__builtin_amdgcn_s_waitcnt(0);
-	__syncthreads();
-	if(threadIdx.x == 0) {
-		__threadfence();
-		atomicAdd(bar, 1u);
-		if(__hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
+	__hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template<typename V>
+__device__ inline V ld_agent(const V* p)
+{
+	return __hip_atomic_load(const_cast<V*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+/* Wave 0 waits until every partition of the network has published its
+ * fired count for cycle t (count tagged with the low 32 bits of t; the
+ * buffer of the same parity last held cycle t-2), then writes the exclusive
+ * prefix of the counts to pref[0..P]. This is the network's only grid-level
+ * synchronisation per cycle. The wait is bounded: after spinLimit polls the
+ * status word is flagged, and a flagged status makes every later wait of the
+ * launch fall through, so all waves run to completion. */
+__device__ inline void gather_counts(const uint64_t* fc, uint32_t P, uint32_t tag, uint32_t* pref,
+		uint32_t lane, uint32_t spinLimit, uint32_t* status)
+{
+	uint32_t carry = 0;
+	for(uint32_t c = 0; c < P; c += 64u) {
+		uint32_t cnt = 0;
+		if(c + lane < P) {
 			uint32_t spins = 0;
-			while(__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+			for(;;) {
+				const uint64_t v = ld_agent(&fc[c + lane]);
+				if(uint32_t(v >> 32) == tag) {
+					cnt = uint32_t(v);
+					break;
+				}
+				if(ld_agent(status) != 0u) {
+					break;
+				}
 				__builtin_amdgcn_s_sleep(1);
 				if(++spins > spinLimit) {
 					atomicOr(status, NEMO_STATUS_BARRIER_TIMEOUT);
@@ -83,9 +109,37 @@ __device__ inline void net_barrier(uint32_t* bar, uint32_t target, uint32_t spin
 				}
 			}
 		}
-		__threadfence();
+		uint32_t x = cnt;
+#pragma unroll
+		for(uint32_t d = 1; d < 64u; d <<= 1) {
+			uint32_t y = __shfl_up(x, d, 64);
+			if(lane >= d) x += y;
+		}
+		if(c + lane < P) pref[c + lane + 1] = carry + x;
+		carry += __shfl(x, 63, 64);
 	}
-	__syncthreads();
+	if(lane == 0) pref[0] = 0u;
+}
+
+/* Add a Q11.20 weight into ring slot sl of local target tl: the excitatory
+ * plane for w >= 0, the inhibitory plane otherwise (as the CPU's split,
+ * src/nemo/cpu/Simulation.cpp:299). In saturating mode the returning atomic
+ * detects the first partial sum to leave int32 and sets a sticky bit. */
+template<bool SAT>
+__device__ inline void ring_add(int32_t* ringE, int32_t* ringI, uint32_t* ovf, uint32_t D, uint32_t T,
+		uint32_t fwords, uint32_t sl, uint32_t tl, int32_t w)
+{
+	int32_t* acc = (w >= 0) ? ringE : ringI;
+	if(SAT) {
+		int32_t old = __hip_atomic_fetch_add(&acc[sl * T + tl], w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+		int64_t sum = int64_t(old) + int64_t(w);
+		if(sum > int64_t(INT32_MAX) || sum < int64_t(INT32_MIN)) {
+			uint32_t plane = (w >= 0) ? 0u : D;
+			atomicOr(&ovf[(plane + sl) * fwords + (tl >> 5)], 1u << (tl & 31u));
+		}
+	} else {
+		__hip_atomic_fetch_add(&acc[sl * T + tl], w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+	}
 }
 
 template<bool SAT>
@@ -117,12 +171,40 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 
 	/* restore the pending-current ring */
 	const int32_t* gring = nd.ring + size_t(p) * 2u * D * T;
+	const bool replay = (nd.flags & NEMO_NET_REPLAY) != 0u;
 	for(uint32_t i = tid; i < 2u * D * T; i += nthreads) {
-		ringE[i] = gring[i];
+		ringE[i] = replay ? 0 : gring[i];
 	}
 	if(SAT) {
 		for(uint32_t i = tid; i < 2u * D * fwords; i += nthreads) {
 			ovf[i] = 0u;
+		}
+	}
+	if(replay) {
+		/* Weights changed (applyStdp) since the ring was filled. NeMo reads a
+		 * weight when the spike arrives (src/nemo/cpu/Simulation.cpp:258-307),
+		 * so rebuild the ring from the firing history: every spike fired at
+		 * cycle0-1-k whose synapse delay d > k has not arrived yet and lands
+		 * in slot (cycle0-1-k+d) mod D, with the current weight. */
+		__syncthreads();
+		const uint64_t pend = (D >= 64u) ? ~0ull : ((1ull << D) - 1ull);
+		const uint32_t last = uint32_t((nd.cycle0 + D - 1u) % D);    /* slot of cycle0-1 */
+		const uint2* syn = reinterpret_cast<const uint2*>(nd.syn);
+		for(uint32_t s = tid; s < n_total; s += nthreads) {
+			const uint64_t h = nd.recent[s] & pend;
+			if(!h) continue;
+			const uint32_t* o = nd.off + size_t(s) * (P + 1u) + p;
+			const uint32_t a = o[0], b = o[1];
+			for(uint32_t q = a; q < b; ++q) {
+				const uint2 rec = syn[q];
+				const uint32_t d = rec.x >> 16;
+				uint64_t hb = h & ((d >= 64u) ? ~0ull : ((1ull << d) - 1ull));
+				while(hb) {
+					const uint32_t kk = uint32_t(__builtin_ctzll(hb));
+					hb &= hb - 1ull;
+					ring_add<SAT>(ringE, ringI, ovf, D, T, fwords, (last + d - kk) % D, rec.x & 0xffffu, int32_t(rec.y));
+				}
+			}
 		}
 	}
 
@@ -208,7 +290,7 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 			fired = update_neuron(model, prm, st, rng, fx20_to_float(e), fx20_to_float(i), cx, fs);
 			recent = (recent << 1) | (fired ? 1ull : 0ull);
 			if(stdp) {
-				nd.rfg[size_t(par) * n_total + n] = recent;
+				st_agent(&nd.rfg[size_t(par) * n_total + n], recent);
 			}
 		}
 
@@ -234,34 +316,23 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 			if(fired) {
 				const uint64_t below = lane ? (mask & (~0ull >> (64u - lane))) : 0ull;
 				const uint32_t pos = base + uint32_t(__popcll(below));
-				nd.fl[(size_t(par) * P + p) * T + pos] = n;
+				st_agent(&nd.fl[(size_t(par) * P + p) * T + pos], n);
 			}
+			/* publish: every wave's fired-list stores complete, then the count */
+			__builtin_amdgcn_s_waitcnt(0);
+			__syncthreads();
 			if(tid == 0) {
-				nd.fc[par * P + p] = total;
+				st_agent(&nd.fc[par * P + p], (uint64_t(uint32_t(t)) << 32) | uint64_t(total));
 			}
 		}
 
-		/* ---- 3. the network's barrier */
-		net_barrier(nd.bar, (k + 1u) * P, nd.spinLimit, nd.status);
-
-		/* ---- 4. delivery of this cycle's spikes into the ring */
+		/* ---- 3. wait for the network's other partitions */
 		if(wave == 0) {
-			uint32_t carry = 0;
-			for(uint32_t c = 0; c < P; c += 64u) {
-				uint32_t v = (c + lane < P) ? nd.fc[par * P + c + lane] : 0u;
-				uint32_t x = v;
-				/* inclusive wave scan */
-#pragma unroll
-				for(uint32_t d = 1; d < 64u; d <<= 1) {
-					uint32_t y = __shfl_up(x, d, 64);
-					if(lane >= d) x += y;
-				}
-				if(c + lane < P) pref[c + lane + 1] = carry + x;
-				carry += __shfl(x, 63, 64);
-			}
-			if(lane == 0) pref[0] = 0u;
+			gather_counts(nd.fc + par * P, P, uint32_t(t), pref, lane, nd.spinLimit, nd.status);
 		}
 		__syncthreads();
+
+		/* ---- 4. delivery of this cycle's spikes into the ring */
 		{
 			const uint32_t F = pref[P];
 			const uint32_t* fl = nd.fl + size_t(par) * P * T;
@@ -273,28 +344,16 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 					uint32_t mid = (lo + hi) >> 1;
 					if(pref[mid] <= j) lo = mid; else hi = mid;
 				}
-				const uint32_t s = fl[size_t(lo) * T + (j - pref[lo])];
+				const uint32_t s = ld_agent(&fl[size_t(lo) * T + (j - pref[lo])]);
 				const uint32_t* o = off + size_t(s) * (P + 1u) + p;
 				const uint32_t a = o[0];
 				const uint32_t b = o[1];
 				events += b - a;
 				for(uint32_t q = a; q < b; ++q) {
 					const uint2 rec = syn[q];
-					const uint32_t tl = rec.x & 0xffffu;
 					uint32_t sl = slot + (rec.x >> 16);
 					sl = (sl >= D) ? sl - D : sl;
-					const int32_t w = int32_t(rec.y);
-					int32_t* acc = (w >= 0) ? ringE : ringI;
-					if(SAT) {
-						int32_t old = __hip_atomic_fetch_add(&acc[sl * T + tl], w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-						int64_t sum = int64_t(old) + int64_t(w);
-						if(sum > int64_t(INT32_MAX) || sum < int64_t(INT32_MIN)) {
-							uint32_t plane = (w >= 0) ? 0u : D;
-							atomicOr(&ovf[(plane + sl) * fwords + (tl >> 5)], 1u << (tl & 31u));
-						}
-					} else {
-						__hip_atomic_fetch_add(&acc[sl * T + tl], w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-					}
+					ring_add<SAT>(ringE, ringI, ovf, D, T, fwords, sl, rec.x & 0xffffu, int32_t(rec.y));
 				}
 			}
 		}
@@ -322,7 +381,7 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 					const uint32_t sd = nd.rcm_sd[r];
 					const uint32_t src = sd & 0x3ffffffu;
 					const uint32_t d = (sd >> 26) + 1u;
-					const uint64_t pre = rf[src] >> d;
+					const uint64_t pre = ld_agent(&rf[src]) >> d;
 					const int32_t wd = stdp_weight_change(*sp, pre);
 					if(wd != 0) {
 						nd.rcm_acc[r] = int32_t(uint32_t(nd.rcm_acc[r]) + uint32_t(wd));

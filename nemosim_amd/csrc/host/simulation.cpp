@@ -401,7 +401,7 @@ Simulation::Simulation(const NetworkImpl& net, const ConfigurationImpl& conf) :
 	d_off.upload(off.data(), off.size() * sizeof(uint32_t));
 	d_syn.upload(syn.data(), syn.size() * sizeof(uint32_t));
 	d_fl.alloc(size_t(2) * m_P * m_T * 4);
-	d_fc.alloc(size_t(2) * m_P * 4 + 4);
+	d_fc.alloc(size_t(2) * m_P * 8 + 8);
 	d_ctl.alloc(64);
 	if(m_stdp) {
 		d_rfg.alloc(size_t(2) * m_n * 8);
@@ -416,6 +416,8 @@ Simulation::Simulation(const NetworkImpl& net, const ConfigurationImpl& conf) :
 	if(d_ring.bytes) hipCheck(nemo_dev_fill_u32(d_ring.as<uint32_t>(), 0, d_ring.bytes / 4, s), "fill");
 	if(d_rfg.bytes) hipCheck(nemo_dev_fill_u64(d_rfg.as<uint64_t>(), 0, size_t(2) * m_n, s), "fill");
 	if(d_rcmAcc.bytes) hipCheck(nemo_dev_fill_u32(d_rcmAcc.as<uint32_t>(), 0, d_rcmAcc.bytes / 4, s), "fill");
+	/* tagged fired counts: all-ones never matches a cycle's tag */
+	if(d_fc.bytes) hipCheck(nemo_dev_fill_u64(d_fc.as<uint64_t>(), ~0ull, d_fc.bytes / 8, s), "fill");
 	hipCheck(nemo_dev_fill_u32(d_ctl.as<uint32_t>(), 0, 16, s), "fill");
 
 	if(m_fused) {
@@ -587,7 +589,8 @@ Simulation::prepareLaunch(unsigned nsteps, void* stream)
 	nd.T = m_T;
 	nd.D = m_D;
 	nd.words = m_words;
-	nd.flags = (m_stdp ? NEMO_NET_STDP : 0u) | (m_saturate ? NEMO_NET_SATURATE : 0u);
+	nd.flags = (m_stdp ? NEMO_NET_STDP : 0u) | (m_saturate ? NEMO_NET_SATURATE : 0u)
+		| (m_replay ? NEMO_NET_REPLAY : 0u);
 	nd.nsteps = nsteps;
 	nd.spinLimit = m_spinLimit;
 	nd.cycle0 = m_cycle;
@@ -600,9 +603,8 @@ Simulation::prepareLaunch(unsigned nsteps, void* stream)
 	nd.off = d_off.as<uint32_t>();
 	nd.syn = d_syn.as<uint32_t>();
 	nd.fl = d_fl.as<uint32_t>();
-	nd.fc = d_fc.as<uint32_t>();
+	nd.fc = d_fc.as<uint64_t>();
 	nd.rfg = d_rfg.as<uint64_t>();
-	nd.bar = d_ctl.as<uint32_t>();
 	nd.status = d_ctl.as<uint32_t>() + 1;
 	nd.events = reinterpret_cast<unsigned long long*>(d_ctl.as<uint32_t>() + 2);
 	nd.rcm_start = d_rcmStart.as<uint32_t>();
@@ -632,6 +634,7 @@ Simulation::finishLaunch(unsigned nsteps, void* stream, unsigned long long* even
 	m_timerCycles += nsteps;
 	m_lastRunSteps = nsteps;
 	m_hasSchedule = false;
+	m_replay = false;
 	if(ctl[1] & NEMO_STATUS_BARRIER_TIMEOUT) {
 		throw exception(NEMO_CUDA_ERROR, "step kernel: a workgroup barrier timed out (workgroups not co-resident?)");
 	}
@@ -686,6 +689,12 @@ Simulation::launchPlugins(unsigned nsteps, unsigned long long* events)
 	hipCheck(hipEventCreate(&e0), "hipEventCreate");
 	hipCheck(hipEventCreate(&e1), "hipEventCreate");
 	hipCheck(hipEventRecord(e0, st), "hipEventRecord");
+	if(m_replay) {
+		/* weights changed: rebuild the pending current from the history */
+		hipCheck(nemo_dev_fill_u64(d_acc.as<uint64_t>(), 0, d_acc.bytes / 8, st), "fill");
+		hipCheck(nemo_dev_pm_replay(&s, st), "plugin step: replay");
+		m_replay = false;
+	}
 	for(unsigned k = 0; k < nsteps; ++k) {
 		hipCheck(nemo_dev_pm_begin(&s, k, st), "plugin step: begin");
 		if(m_hasSchedule) {
@@ -872,6 +881,8 @@ Simulation::applyStdp(float reward)
 	hipCheck(nemo_dev_apply_stdp(d_syn.as<uint32_t>(), d_rcmFwd.as<uint32_t>(), d_rcmAcc.as<int32_t>(),
 				m_nplastic, d_stdp.ptr, fxr, st), "applyStdp kernel");
 	hipCheck(hipStreamSynchronize(st), "hipStreamSynchronize");
+	/* spikes in flight now arrive with the new weights */
+	m_replay = true;
 }
 
 void

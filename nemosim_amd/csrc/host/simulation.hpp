@@ -162,6 +162,8 @@ class Simulation
 
 		/* schedule of the current launch (host side, sim-local indices) */
 		bool m_hasSchedule = false;
+		/* weights changed since the pending-current ring was filled */
+		bool m_replay = false;
 		unsigned m_lastRunSteps = 0;
 
 		std::vector<unsigned> m_fired;

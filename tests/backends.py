@@ -24,7 +24,7 @@ class OracleBuilder(object):
             self.o.add_neurons(name, idx, args)
         self.o.add_synapses_csr(spec.rowptr, spec.targets, spec.delays, spec.weights, spec.plastic)
 
-    def simulation(self, partition_size=None):
+    def simulation(self, partition_size=None, step_mode=None):
         self.o.finalize()
         return self.o
 
@@ -56,9 +56,11 @@ class HipBuilder(object):
             self.net.add_neurons(self.types[name], idx, args)
         self.net.add_synapses_csr(spec.rowptr, spec.targets, spec.delays, spec.weights, spec.plastic)
 
-    def simulation(self, partition_size=None):
+    def simulation(self, partition_size=None, step_mode=None):
         if partition_size:
             self.conf.set_partition_size(partition_size)
+        if step_mode:
+            self.conf.set_step_mode(step_mode)
         return self.nemo.Simulation(self.net, self.conf)
 
 

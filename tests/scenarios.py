@@ -231,3 +231,52 @@ def stdp_closed_form(make, reward=1.0, noise=False, epochs=4):
         got = expect[e]
         for i in range(n):
             assert got[i] == np.float32(np.float32(w[i]) / np.float32(1 << 20)), (e, i, got[i], w[i])
+
+
+def stdp_in_flight(make, delay=20, reward=1.0, **sim_args):
+    """A spike in flight across applyStdp arrives with the new weight: NeMo
+    reads a synapse's weight when the spike is delivered
+    (src/nemo/cpu/Simulation.cpp:258-307), not when the source fires.
+
+    Source S forced at cycle 10 (arrival 10+delay), integrator X forced at
+    cycle 35 (so the pre-post pair updates the plastic synapse S->X by
+    reward*fnPre[35-(10+delay)]), S forced again at cycle 90, applyStdp after
+    cycle 99, and the second spike arrives at cycle 90+delay > 100. X
+    integrates exactly (tau_syn=1, no leak, tau_refrac=0), so its potential
+    steps by the converted new weight at the arrival cycle."""
+    assert 10 + delay <= 35 and 90 + delay >= 100
+    S, X = 0, 1
+    w0 = 0.5
+    pre_fn = [np.float32(1.0) * np.float32(np.exp(-dt / 20.0)) for dt in range(1, 21)]
+    post_fn = [np.float32(-0.8) * np.float32(np.exp(-dt / 20.0)) for dt in range(1, 21)]
+    b = make()
+    b.set_stdp(pre_fn, post_fn, 0.0, 100.0, 0.0, -100.0)
+    netutil.if_source(b, S)
+    netutil.if_integrator(b, X)
+    sid = b.add_synapse(S, X, delay, w0, 1)
+    sim = b.simulation(**sim_args)
+    arrive = 90 + delay
+    v_before = None
+    for c in range(arrive + 1):
+        fs = []
+        if c in (10, 90):
+            fs = [S]
+        if c == 35:
+            fs = [X]
+        if c == 100:
+            sim.apply_stdp(reward)
+            w1 = sim.get_synapse_weight(sid)
+        sim.step(fs)
+        if c == arrive - 1:
+            v_before = sim.get_neuron_state(X, 0)
+    # closed form of the updated weight (StdpProcess.cpp:87-113, ConnectivityMatrix.cpp:247-276)
+    dwf = pre_fn[35 - (10 + delay)]
+    delta = (netutil.fix20(reward) * netutil.fix20(dwf)) >> 20
+    w1_fx = min(netutil.fix20(100.0), max(netutil.fix20(w0) + delta, 0))
+    assert w1 == np.float32(np.float32(w1_fx) / np.float32(1 << 20)), (w1, w1_fx)
+    assert w1_fx != netutil.fix20(w0)
+    cur = np.float32(netutil.sat_to_float(w1_fx) + netutil.sat_to_float(0))
+    cur = np.float32(np.float32(cur + np.float32(0.0)) + np.float32(0.0))
+    expect = np.float32(v_before + np.float32(np.float32(cur / np.float32(1.0)) + np.float32(0.0)))
+    got = sim.get_neuron_state(X, 0)
+    assert got == expect, (got, expect, v_before)

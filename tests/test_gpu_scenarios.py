@@ -55,3 +55,9 @@ def test_integrator_brute_force_saturating(make):
 @pytest.mark.parametrize("reward,noise", [(1.0, False), (0.8, False), (0.9, True)])
 def test_stdp_closed_form(make, reward, noise):
     scenarios.stdp_closed_form(make, reward=reward, noise=noise)
+
+
+@pytest.mark.parametrize("delay", [20, 25])
+@pytest.mark.parametrize("step_mode", ["fused", "plugins"])
+def test_stdp_in_flight(make, delay, step_mode):
+    scenarios.stdp_in_flight(make, delay=delay, step_mode=step_mode)

@@ -74,3 +74,8 @@ def test_apply_stdp_without_stdp(oracle_mod):
     with pytest.raises(oracle_mod.OracleError) as e:
         o.apply_stdp(1.0)
     assert e.value.status == 9
+
+
+@pytest.mark.parametrize("delay", [20, 25])
+def test_stdp_in_flight(make, delay):
+    scenarios.stdp_in_flight(make, delay=delay)

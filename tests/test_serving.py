@@ -1,0 +1,48 @@
+"""Serving: continuous batching under an HBM budget gives every request the
+same raster as running it alone (and the oracle's), with STDP requests
+applying their reward on their own schedule."""
+import numpy as np
+import pytest
+
+from backends import first_divergence
+from nemosim_amd import serving, examples
+
+
+def test_budget_admission_order_cpu():
+    """Admission is FIFO by arrival and never exceeds the budget (host
+    logic, no GPU)."""
+    reqs = serving.poisson_requests(20, rate_hz=50.0, seed=3, size_range=(1000, 5000), syn_range=(50, 100),
+                                    dur_range=(50, 100))
+    assert [r.rid for r in sorted(reqs, key=lambda r: r.arrival_s)] == list(range(20))
+    for r in reqs:
+        assert r.estimated_device_bytes() > 8 * r.neurons * r.synapses
+    too_big = serving.Request(0, 0.0, 50000, 1000, 10)
+    srv = serving.Server(hbm_budget_bytes=10 << 20)
+    with pytest.raises(ValueError):
+        srv.serve([too_big], realtime=False)
+
+
+@pytest.mark.gpu
+def test_alone_equals_mix(oracle_mod):
+    reqs = [
+        serving.Request(0, 0.00, 1500, 100, 350, seed=31),
+        serving.Request(1, 0.00, 4000, 80, 250, model="IF_curr_exp_noise", seed=32),
+        serving.Request(2, 0.01, 2500, 100, 1300, stdp=True, seed=33, stdp_period=500),
+        serving.Request(3, 0.02, 1200, 60, 420, seed=34),
+    ]
+    # a budget that holds only some of them at once, so admission is staggered
+    budget = reqs[0].estimated_device_bytes() + reqs[1].estimated_device_bytes() + reqs[2].estimated_device_bytes()
+    srv = serving.Server(hbm_budget_bytes=budget, chunk_cycles=64, build_workers=2)
+    stats = srv.serve(reqs, realtime=False)
+    assert stats["requests"] == 4
+    assert stats["p99_latency_s"] >= stats["p50_latency_s"] > 0
+    for r in reqs:
+        got_c, got_i = serving.raster_of(r)
+        alone_c, alone_i = serving.run_alone(r)
+        d = first_divergence(alone_c, alone_i, got_c, got_i)
+        assert d is None, "request %d: mix vs alone, cycle %d: %s" % ((r.rid,) + d)
+        if r.model == "Izhikevich" and not r.stdp:
+            orc = oracle_mod.from_spec(r.spec())
+            rc, ri = orc.run(r.duration_ms)
+            d = first_divergence(rc, ri, got_c, got_i)
+            assert d is None, "request %d vs oracle, cycle %d: %s" % ((r.rid,) + d)
