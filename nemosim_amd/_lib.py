@@ -1,13 +1,15 @@
 """Loader of libnemo_hip.so, the HIP backend with NeMo's C API.
 
 The library is built in-tree (``make -C nemosim_amd``); importing this module
-without it raises ImportError -- there is no CPU fallback.
+without it raises ImportError -- there is no CPU fallback. The environment
+variable NEMOSIM_AMD_LIBRARY names another build of the same library (for
+example one compiled with different kernel options) to load instead.
 """
 import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libnemo_hip.so")
+LIB_PATH = os.environ.get("NEMOSIM_AMD_LIBRARY") or os.path.join(_HERE, "libnemo_hip.so")
 
 c_uint_p = ctypes.POINTER(ctypes.c_uint)
 c_float_p = ctypes.POINTER(ctypes.c_float)

@@ -36,6 +36,8 @@ typedef struct NetDev {
 	uint32_t flags;      /* NEMO_NET_* */
 	uint32_t nsteps;     /* steps this network runs in the current launch */
 	uint32_t spinLimit;  /* barrier spin bound (iterations) */
+	uint32_t glog2;      /* log2 of the lanes that share one fired source in delivery (0..6) */
+	uint32_t pad0;
 	uint64_t cycle0;     /* cycle count at the start of the launch */
 
 	const uint8_t* model;     /* [n] model id */
@@ -87,6 +89,13 @@ typedef struct WorkItem {
 int nemo_dev_step_launch(const NetDev* nets_dev, const WorkItem* work_dev, uint32_t nblocks,
 		uint32_t blockDim, uint32_t ldsBytes, void* stream);
 int nemo_dev_step_max_blocks(uint32_t blockDim, uint32_t ldsBytes, uint32_t* maxBlocks);
+/* Batched step as an instantiated HIP graph (clear -> cooperative step ->
+ * collect); collect_dev receives [nnets][4] words {status, 0, events lo, hi}.
+ * The graph reads nets_dev/work_dev at replay time. */
+int nemo_dev_batch_graph_create(const NetDev* nets_dev, uint32_t nnets, const WorkItem* work_dev,
+		uint32_t nblocks, uint32_t blockDim, uint32_t ldsBytes, uint32_t* collect_dev, void** exec_out);
+int nemo_dev_graph_launch(void* exec, void* stream);
+int nemo_dev_graph_destroy(void* exec);
 int nemo_dev_apply_stdp(uint32_t* syn, const uint32_t* rcm_fwd, int32_t* rcm_acc, size_t nr,
 		const void* stdp_dev, int32_t fx_reward, void* stream);
 int nemo_dev_fill_u32(uint32_t* p, uint32_t value, size_t n, void* stream);

@@ -66,16 +66,32 @@ __device__ inline uint32_t lds_prefix_waves(uint32_t* wsum, uint32_t wave, uint3
  * write-back or invalidate is needed. A writer drains its stores
  * (s_waitcnt; __syncthreads alone waits only for LDS traffic) before
  * publishing the count that names them. */
+#if defined(__HIP_DEVICE_COMPILE__)
+#define NEMO_GLOBAL __attribute__((address_space(1)))
+#else
+#define NEMO_GLOBAL   /* host pass: the kernels are only parsed there */
+#endif
+
+/* The network's arrays as global-address-space pointers. Pointers read from
+ * the network descriptor are generic, and generic (flat) accesses also count
+ * against the LDS wait counter, so every LDS wait would also wait for all
+ * outstanding memory loads; global accesses keep the two independent. */
 template<typename V>
-__device__ inline void st_agent(V* p, V v)
+__device__ inline NEMO_GLOBAL V* glb(V* p)
+{
+	return (NEMO_GLOBAL V*) p;
+}
+
+template<typename V>
+__device__ inline void st_agent(NEMO_GLOBAL V* p, V v)
 {
 	__hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 template<typename V>
-__device__ inline V ld_agent(const V* p)
+__device__ inline V ld_agent(const NEMO_GLOBAL V* p)
 {
-	return __hip_atomic_load(const_cast<V*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+	return __hip_atomic_load(const_cast<NEMO_GLOBAL V*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 /* Wave 0 waits until every partition of the network has published its
@@ -85,7 +101,7 @@ __device__ inline V ld_agent(const V* p)
  * synchronisation per cycle. The wait is bounded: after spinLimit polls the
  * status word is flagged, and a flagged status makes every later wait of the
  * launch fall through, so all waves run to completion. */
-__device__ inline void gather_counts(const uint64_t* fc, uint32_t P, uint32_t tag, uint32_t* pref,
+__device__ inline void gather_counts(const NEMO_GLOBAL uint64_t* fc, uint32_t P, uint32_t tag, uint32_t* pref,
 		uint32_t lane, uint32_t spinLimit, uint32_t* status)
 {
 	uint32_t carry = 0;
@@ -99,7 +115,7 @@ __device__ inline void gather_counts(const uint64_t* fc, uint32_t P, uint32_t ta
 					cnt = uint32_t(v);
 					break;
 				}
-				if(ld_agent(status) != 0u) {
+				if(ld_agent(glb(status)) != 0u) {
 					break;
 				}
 				__builtin_amdgcn_s_sleep(1);
@@ -119,6 +135,21 @@ __device__ inline void gather_counts(const uint64_t* fc, uint32_t P, uint32_t ta
 		carry += __shfl(x, 63, 64);
 	}
 	if(lane == 0) pref[0] = 0u;
+}
+
+/* Rounds of record loads issued together in the delivery loop */
+constexpr uint32_t DELIVER_RU = 16u;
+
+/* Partition whose fired list holds flat index j: the q with
+ * pref[q] <= j < pref[q+1] (pref is the exclusive prefix of counts) */
+__device__ inline uint32_t partition_of(const uint32_t* pref, uint32_t P, uint32_t j)
+{
+	uint32_t lo = 0, hi = P;
+	while(hi - lo > 1u) {
+		const uint32_t mid = (lo + hi) >> 1;
+		if(pref[mid] <= j) lo = mid; else hi = mid;
+	}
+	return lo;
 }
 
 /* Add a Q11.20 weight into ring slot sl of local target tl: the excitatory
@@ -189,11 +220,11 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 		__syncthreads();
 		const uint64_t pend = (D >= 64u) ? ~0ull : ((1ull << D) - 1ull);
 		const uint32_t last = uint32_t((nd.cycle0 + D - 1u) % D);    /* slot of cycle0-1 */
-		const uint2* syn = reinterpret_cast<const uint2*>(nd.syn);
+		const NEMO_GLOBAL uint2* syn = glb(reinterpret_cast<const uint2*>(nd.syn));
 		for(uint32_t s = tid; s < n_total; s += nthreads) {
-			const uint64_t h = nd.recent[s] & pend;
+			const uint64_t h = glb(nd.recent)[s] & pend;
 			if(!h) continue;
-			const uint32_t* o = nd.off + size_t(s) * (P + 1u) + p;
+			const NEMO_GLOBAL uint32_t* o = glb(nd.off) + size_t(s) * (P + 1u) + p;
 			const uint32_t a = o[0], b = o[1];
 			for(uint32_t q = a; q < b; ++q) {
 				const uint2 rec = syn[q];
@@ -244,8 +275,8 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 		/* ---- stimulus of this step (sim-local indices, deduplicated by the host) */
 		bool has_stim = false;
 		uint32_t fs0 = 0, fs1 = 0, is0 = 0, is1 = 0;
-		if(nd.fs_ptr) { fs0 = nd.fs_ptr[k]; fs1 = nd.fs_ptr[k + 1]; }
-		if(nd.is_ptr) { is0 = nd.is_ptr[k]; is1 = nd.is_ptr[k + 1]; }
+		if(nd.fs_ptr) { fs0 = glb(nd.fs_ptr)[k]; fs1 = glb(nd.fs_ptr)[k + 1]; }
+		if(nd.is_ptr) { is0 = glb(nd.is_ptr)[k]; is1 = glb(nd.is_ptr)[k + 1]; }
 		has_stim = (fs1 > fs0) || (is1 > is0);
 		if(has_stim) {
 			if(tid < T) ext[tid] = 0.0f;
@@ -253,16 +284,16 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 			__syncthreads();
 			const uint32_t lo = p * T, hi = p * T + T;
 			for(uint32_t e = fs0 + tid; e < fs1; e += nthreads) {
-				uint32_t idx = nd.fs_idx[e];
+				uint32_t idx = glb(nd.fs_idx)[e];
 				if(idx >= lo && idx < hi) {
 					uint32_t l = idx - lo;
 					atomicOr(&fst[l >> 5], 1u << (l & 31u));
 				}
 			}
 			for(uint32_t e = is0 + tid; e < is1; e += nthreads) {
-				uint32_t idx = nd.is_idx[e];
+				uint32_t idx = glb(nd.is_idx)[e];
 				if(idx >= lo && idx < hi) {
-					ext[idx - lo] = nd.is_val[e];
+					ext[idx - lo] = glb(nd.is_val)[e];
 				}
 			}
 			__syncthreads();
@@ -290,7 +321,7 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 			fired = update_neuron(model, prm, st, rng, fx20_to_float(e), fx20_to_float(i), cx, fs);
 			recent = (recent << 1) | (fired ? 1ull : 0ull);
 			if(stdp) {
-				st_agent(&nd.rfg[size_t(par) * n_total + n], recent);
+				st_agent(glb(nd.rfg) + size_t(par) * n_total + n, recent);
 			}
 		}
 
@@ -299,7 +330,7 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 		if(lane == 0) {
 			wsum[wave] = uint32_t(__popcll(mask));
 			if(wave < (T >> 6)) {
-				nd.raster[size_t(k) * nd.words + p * (T >> 6) + wave] = mask;
+				glb(nd.raster)[size_t(k) * nd.words + p * (T >> 6) + wave] = mask;
 			}
 		}
 		__syncthreads();
@@ -316,44 +347,119 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 			if(fired) {
 				const uint64_t below = lane ? (mask & (~0ull >> (64u - lane))) : 0ull;
 				const uint32_t pos = base + uint32_t(__popcll(below));
-				st_agent(&nd.fl[(size_t(par) * P + p) * T + pos], n);
+				st_agent(glb(nd.fl) + (size_t(par) * P + p) * T + pos, n);
 			}
 			/* publish: every wave's fired-list stores complete, then the count */
 			__builtin_amdgcn_s_waitcnt(0);
 			__syncthreads();
 			if(tid == 0) {
-				st_agent(&nd.fc[par * P + p], (uint64_t(uint32_t(t)) << 32) | uint64_t(total));
+				st_agent(glb(nd.fc) + par * P + p, (uint64_t(uint32_t(t)) << 32) | uint64_t(total));
 			}
 		}
 
 		/* ---- 3. wait for the network's other partitions */
 		if(wave == 0) {
-			gather_counts(nd.fc + par * P, P, uint32_t(t), pref, lane, nd.spinLimit, nd.status);
+			gather_counts(glb(nd.fc) + par * P, P, uint32_t(t), pref, lane, nd.spinLimit, nd.status);
 		}
 		__syncthreads();
 
 		/* ---- 4. delivery of this cycle's spikes into the ring */
 		{
 			const uint32_t F = pref[P];
-			const uint32_t* fl = nd.fl + size_t(par) * P * T;
-			const uint32_t* off = nd.off;
-			const uint2* syn = reinterpret_cast<const uint2*>(nd.syn);
-			for(uint32_t j = tid; j < F; j += nthreads) {
-				uint32_t lo = 0, hi = P;
-				while(hi - lo > 1u) {
-					uint32_t mid = (lo + hi) >> 1;
-					if(pref[mid] <= j) lo = mid; else hi = mid;
+			const NEMO_GLOBAL uint32_t* fl = glb(nd.fl) + size_t(par) * P * T;
+			const NEMO_GLOBAL uint2* syn = glb(reinterpret_cast<const uint2*>(nd.syn));
+			const NEMO_GLOBAL uint32_t* off = glb(nd.off);
+			/* Each wave takes a contiguous share of the F fired sources and
+			 * first loads, for up to 128 of them at once, the fired-list entry
+			 * and the bucket bounds off[s][p..p+1] (one lane per source, all
+			 * loads in flight together). Then groups of G lanes share one
+			 * source each: lane i of a group reads records i, i+G, ... of the
+			 * bucket, so one load instruction touches a few cache lines
+			 * instead of 64, and up to DELIVER_RU rounds of record loads are
+			 * issued before their LDS atomics. G is the host's choice per
+			 * network (about 1.5x the mean bucket length). */
+			const uint32_t glog = nd.glog2;
+			const uint32_t G = 1u << glog;
+			const uint32_t NG = 64u >> glog;                 /* sources per round */
+			const uint32_t grp = lane >> glog;
+			const uint32_t gl = lane & (G - 1u);
+			const uint32_t share = (F + nwaves - 1u) / nwaves;
+			const uint32_t jw0 = min(F, wave * share);
+			const uint32_t jw1 = min(F, jw0 + share);
+			for(uint32_t jb = jw0; jb < jw1; jb += 128u) {   /* wave-uniform */
+				const uint32_t nsrc = min(128u, jw1 - jb);
+				uint32_t a0 = 0, b0 = 0, a1 = 0, b1 = 0;
+				{
+					const uint32_t j0 = jb + lane;
+					const uint32_t j1 = jb + 64u + lane;
+					const bool v0 = lane < nsrc;
+					const bool v1 = 64u + lane < nsrc;
+					const uint32_t q0 = v0 ? partition_of(pref, P, j0) : 0u;
+					const uint32_t q1 = v1 ? partition_of(pref, P, j1) : 0u;
+					uint32_t s0 = 0, s1 = 0;
+					if(v0) s0 = ld_agent(&fl[size_t(q0) * T + (j0 - pref[q0])]);
+					if(v1) s1 = ld_agent(&fl[size_t(q1) * T + (j1 - pref[q1])]);
+					if(v0) {
+						uint2 ab;
+						const NEMO_GLOBAL uint32_t* o = off + size_t(s0) * (P + 1u) + p;
+						ab.x = o[0];
+						ab.y = o[1];
+						a0 = ab.x;
+						b0 = ab.y;
+					}
+					if(v1) {
+						uint2 ab;
+						const NEMO_GLOBAL uint32_t* o = off + size_t(s1) * (P + 1u) + p;
+						ab.x = o[0];
+						ab.y = o[1];
+						a1 = ab.x;
+						b1 = ab.y;
+					}
 				}
-				const uint32_t s = ld_agent(&fl[size_t(lo) * T + (j - pref[lo])]);
-				const uint32_t* o = off + size_t(s) * (P + 1u) + p;
-				const uint32_t a = o[0];
-				const uint32_t b = o[1];
-				events += b - a;
-				for(uint32_t q = a; q < b; ++q) {
-					const uint2 rec = syn[q];
-					uint32_t sl = slot + (rec.x >> 16);
-					sl = (sl >= D) ? sl - D : sl;
-					ring_add<SAT>(ringE, ringI, ovf, D, T, fwords, sl, rec.x & 0xffffu, int32_t(rec.y));
+				events += (b0 - a0) + (b1 - a1);
+				const uint32_t rounds = (nsrc + NG - 1u) >> (6u - glog);
+				const bool anyLong = (__ballot((b0 - a0 > G) || (b1 - a1 > G)) != 0ull);
+				for(uint32_t r0 = 0; r0 < rounds; r0 += DELIVER_RU) {    /* wave-uniform */
+					uint2 rec[DELIVER_RU];
+					uint32_t ok = 0;
+#pragma unroll
+					for(uint32_t u = 0; u < DELIVER_RU; ++u) {
+						const uint32_t k = (r0 + u) * NG + grp;      /* source within the batch */
+						const bool hi = ((r0 + u) * NG) >= 64u;     /* wave-uniform */
+						const uint32_t ak = __shfl(hi ? a1 : a0, int(k & 63u), 64);
+						const uint32_t bk = __shfl(hi ? b1 : b0, int(k & 63u), 64);
+						const uint32_t q = ak + gl;
+						const bool valid = (r0 + u < rounds) && (k < nsrc) && (q < bk);
+						/* unconditional load (record 0 stands in for absent
+						 * ones) so no load is left pending on a skipped path */
+						rec[u] = syn[valid ? q : 0u];
+						ok |= uint32_t(valid) << u;
+					}
+#pragma unroll
+					for(uint32_t u = 0; u < DELIVER_RU; ++u) {
+						if((ok >> u) & 1u) {
+							uint32_t sl = slot + (rec[u].x >> 16);
+							sl = (sl >= D) ? sl - D : sl;
+							ring_add<SAT>(ringE, ringI, ovf, D, T, fwords, sl, rec[u].x & 0xffffu, int32_t(rec[u].y));
+						}
+					}
+					if(anyLong) {
+						/* records G, 2G, ... of buckets longer than G */
+						for(uint32_t u = 0; u < DELIVER_RU && r0 + u < rounds; ++u) {
+							const uint32_t k = (r0 + u) * NG + grp;
+							const bool hi = ((r0 + u) * NG) >= 64u;
+							const uint32_t ak = __shfl(hi ? a1 : a0, int(k & 63u), 64);
+							const uint32_t bk = __shfl(hi ? b1 : b0, int(k & 63u), 64);
+							if(k < nsrc) {
+								for(uint32_t q = ak + G + gl; q < bk; q += G) {
+									const uint2 r = syn[q];
+									uint32_t sl = slot + (r.x >> 16);
+									sl = (sl >= D) ? sl - D : sl;
+									ring_add<SAT>(ringE, ringI, ovf, D, T, fwords, sl, r.x & 0xffffu, int32_t(r.y));
+								}
+							}
+						}
+					}
 				}
 			}
 		}
@@ -372,19 +478,19 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 				qlist[base + uint32_t(__popcll(below))] = n;
 			}
 			__syncthreads();
-			const uint64_t* rf = nd.rfg + size_t(par) * n_total;
+			const NEMO_GLOBAL uint64_t* rf = glb(nd.rfg) + size_t(par) * n_total;
 			for(uint32_t qi = 0; qi < nq; ++qi) {
 				const uint32_t tgt = qlist[qi];
-				const uint32_t r0 = nd.rcm_start[tgt];
-				const uint32_t r1 = nd.rcm_start[tgt + 1];
+				const uint32_t r0 = glb(nd.rcm_start)[tgt];
+				const uint32_t r1 = glb(nd.rcm_start)[tgt + 1];
 				for(uint32_t r = r0 + tid; r < r1; r += nthreads) {
-					const uint32_t sd = nd.rcm_sd[r];
+					const uint32_t sd = glb(nd.rcm_sd)[r];
 					const uint32_t src = sd & 0x3ffffffu;
 					const uint32_t d = (sd >> 26) + 1u;
 					const uint64_t pre = ld_agent(&rf[src]) >> d;
 					const int32_t wd = stdp_weight_change(*sp, pre);
 					if(wd != 0) {
-						nd.rcm_acc[r] = int32_t(uint32_t(nd.rcm_acc[r]) + uint32_t(wd));
+						glb(nd.rcm_acc)[r] = int32_t(uint32_t(glb(nd.rcm_acc)[r]) + uint32_t(wd));
 					}
 				}
 			}
@@ -437,6 +543,30 @@ nemo_step_kernel(const NetDev* __restrict__ nets, const WorkItem* __restrict__ w
 		run_partition<true>(nd, wi.part);
 	} else {
 		run_partition<false>(nd, wi.part);
+	}
+}
+
+/* Batched launch, first node: clear each network's status word and event
+ * counter (the step kernel only ORs and adds into them). */
+__global__ void nemo_batch_clear_kernel(const NetDev* __restrict__ nets, uint32_t count)
+{
+	for(uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+		*nets[i].status = 0u;
+		*nets[i].events = 0ull;
+	}
+}
+
+/* Batched launch, last node: gather each network's status and event count
+ * into one array, out[4*i] = status, out[4*i+2..3] = events, so the host
+ * reads all of them with one copy. */
+__global__ void nemo_batch_collect_kernel(const NetDev* __restrict__ nets, uint32_t count, uint32_t* __restrict__ out)
+{
+	for(uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+		const unsigned long long e = *nets[i].events;
+		out[4 * i] = *nets[i].status;
+		out[4 * i + 1] = 0u;
+		out[4 * i + 2] = uint32_t(e);
+		out[4 * i + 3] = uint32_t(e >> 32);
 	}
 }
 
@@ -507,6 +637,76 @@ int nemo_dev_step_launch(const NetDev* nets_dev, const WorkItem* work_dev, uint3
 	e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&nemo_step_kernel),
 			dim3(nblocks), dim3(blockDim), args, ldsBytes, static_cast<hipStream_t>(stream));
 	return int(e);
+}
+
+/* The batched step as a HIP graph: clear -> step (cooperative) -> collect.
+ * Everything a replay needs is read from the device tables nets_dev and
+ * work_dev, so one instantiated graph serves every launch of the same
+ * work table; the host rewrites nets_dev before each replay. */
+int nemo_dev_batch_graph_create(const NetDev* nets_dev, uint32_t nnets, const WorkItem* work_dev,
+		uint32_t nblocks, uint32_t blockDim, uint32_t ldsBytes, uint32_t* collect_dev, void** exec_out)
+{
+	*exec_out = nullptr;
+	hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&nemo_step_kernel),
+			hipFuncAttributeMaxDynamicSharedMemorySize, int(ldsBytes));
+	if(e != hipSuccess) return int(e);
+	hipGraph_t g = nullptr;
+	e = hipGraphCreate(&g, 0);
+	if(e != hipSuccess) return int(e);
+	const uint32_t small = (nnets + 255u) / 256u;
+
+	void* clearArgs[] = { const_cast<NetDev**>(&nets_dev), &nnets };
+	hipKernelNodeParams kp = {};
+	kp.func = reinterpret_cast<void*>(&nemo_batch_clear_kernel);
+	kp.gridDim = dim3(small);
+	kp.blockDim = dim3(256);
+	kp.sharedMemBytes = 0;
+	kp.kernelParams = clearArgs;
+	kp.extra = nullptr;
+	hipGraphNode_t nClear = nullptr, nStep = nullptr, nCollect = nullptr;
+	e = hipGraphAddKernelNode(&nClear, g, nullptr, 0, &kp);
+
+	void* stepArgs[] = { const_cast<NetDev**>(&nets_dev), const_cast<WorkItem**>(&work_dev) };
+	if(e == hipSuccess) {
+		kp.func = reinterpret_cast<void*>(&nemo_step_kernel);
+		kp.gridDim = dim3(nblocks);
+		kp.blockDim = dim3(blockDim);
+		kp.sharedMemBytes = ldsBytes;
+		kp.kernelParams = stepArgs;
+		e = hipGraphAddKernelNode(&nStep, g, &nClear, 1, &kp);
+	}
+	if(e == hipSuccess) {
+		/* all workgroups must be co-resident: they exchange fired lists */
+		hipKernelNodeAttrValue coop = {};
+		coop.cooperative = 1;
+		e = hipGraphKernelNodeSetAttribute(nStep, hipKernelNodeAttributeCooperative, &coop);
+	}
+	void* collectArgs[] = { const_cast<NetDev**>(&nets_dev), &nnets, &collect_dev };
+	if(e == hipSuccess) {
+		kp.func = reinterpret_cast<void*>(&nemo_batch_collect_kernel);
+		kp.gridDim = dim3(small);
+		kp.blockDim = dim3(256);
+		kp.sharedMemBytes = 0;
+		kp.kernelParams = collectArgs;
+		e = hipGraphAddKernelNode(&nCollect, g, &nStep, 1, &kp);
+	}
+	hipGraphExec_t ex = nullptr;
+	if(e == hipSuccess) {
+		e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+	}
+	(void) hipGraphDestroy(g);
+	if(e == hipSuccess) *exec_out = ex;
+	return int(e);
+}
+
+int nemo_dev_graph_launch(void* exec, void* stream)
+{
+	return int(hipGraphLaunch(static_cast<hipGraphExec_t>(exec), static_cast<hipStream_t>(stream)));
+}
+
+int nemo_dev_graph_destroy(void* exec)
+{
+	return exec ? int(hipGraphExecDestroy(static_cast<hipGraphExec_t>(exec))) : 0;
 }
 
 int nemo_dev_step_max_blocks(uint32_t blockDim, uint32_t ldsBytes, uint32_t* maxBlocks)

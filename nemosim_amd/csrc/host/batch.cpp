@@ -1,0 +1,201 @@
+/*
+ * batch.cpp -- the batched form of the step (nemo_hip_run_batch).
+ *
+ * A pack of networks becomes one cooperative launch whose work table maps
+ * each workgroup to (network, partition), XCD-aware over the whole pack.
+ * The launch is a HIP graph of three kernels -- clear the networks' status
+ * words and event counters, the step kernel, gather status and events into
+ * one array -- instantiated once per work table and replayed: every
+ * per-launch value (cycle, step count, stimulus, flags) lives in the
+ * device-side network table, rewritten before each replay with one copy
+ * from pinned host memory. A replay therefore costs one copy in, one graph
+ * launch, one copy out and one synchronisation, whatever the pack size.
+ */
+#include "batch.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <list>
+#include <memory>
+#include <mutex>
+#include <utility>
+#include <vector>
+
+#include <hip/hip_runtime.h>
+
+#include "common.hpp"
+#include "simulation.hpp"
+
+namespace nemo {
+namespace hip {
+
+namespace {
+
+struct BatchPlan {
+	int device = 0;
+	uint32_t blockDim = 0;
+	uint32_t lds = 0;
+	uint32_t nnets = 0;
+	std::vector<WorkItem> work;
+	DevBuf dnets, dwork, dcollect;
+	NetDev* hnets = nullptr;       /* pinned */
+	uint32_t* hcollect = nullptr;  /* pinned, [nnets][4] */
+	void* exec = nullptr;
+
+	bool matches(int dev, uint32_t bd, uint32_t ld, uint32_t nn, const std::vector<WorkItem>& w) const
+	{
+		if(dev != device || bd != blockDim || ld != lds || nn != nnets || w.size() != work.size()) return false;
+		for(size_t i = 0; i < w.size(); ++i) {
+			if(w[i].net != work[i].net || w[i].part != work[i].part) return false;
+		}
+		return true;
+	}
+
+	~BatchPlan()
+	{
+		(void) nemo_dev_graph_destroy(exec);
+		if(hnets) (void) hipHostFree(hnets);
+		if(hcollect) (void) hipHostFree(hcollect);
+	}
+};
+
+/* Instantiated plans, most recently used first. Kept for the life of the
+ * process (never destroyed at exit, when the runtime may be gone). */
+std::mutex g_planMutex;
+std::list<std::unique_ptr<BatchPlan> >* g_plans = new std::list<std::unique_ptr<BatchPlan> >();
+const size_t MAX_PLANS = 32;
+hipStream_t g_streams[64] = {};
+
+hipStream_t
+batchStream(int dev)
+{
+	if(dev < 0 || dev >= 64) throw exception(NEMO_INVALID_INPUT, "device index out of range");
+	if(!g_streams[dev]) {
+		hipCheck(hipStreamCreate(&g_streams[dev]), "hipStreamCreate");
+	}
+	return g_streams[dev];
+}
+
+BatchPlan&
+getPlan(int dev, uint32_t blockDim, uint32_t lds, uint32_t nnets, const std::vector<WorkItem>& work)
+{
+	for(auto it = g_plans->begin(); it != g_plans->end(); ++it) {
+		if((*it)->matches(dev, blockDim, lds, nnets, work)) {
+			g_plans->splice(g_plans->begin(), *g_plans, it);
+			return *g_plans->front();
+		}
+	}
+	std::unique_ptr<BatchPlan> plan(new BatchPlan());
+	plan->device = dev;
+	plan->blockDim = blockDim;
+	plan->lds = lds;
+	plan->nnets = nnets;
+	plan->work = work;
+	plan->dnets.alloc(size_t(nnets) * sizeof(NetDev));
+	plan->dwork.upload(work.data(), work.size() * sizeof(WorkItem));
+	plan->dcollect.alloc(size_t(nnets) * 16);
+	hipCheck(hipHostMalloc(reinterpret_cast<void**>(&plan->hnets), size_t(nnets) * sizeof(NetDev)), "hipHostMalloc");
+	hipCheck(hipHostMalloc(reinterpret_cast<void**>(&plan->hcollect), size_t(nnets) * 16), "hipHostMalloc");
+	hipCheck(nemo_dev_batch_graph_create(plan->dnets.as<NetDev>(), nnets, plan->dwork.as<WorkItem>(),
+				uint32_t(work.size()), blockDim, lds, plan->dcollect.as<uint32_t>(), &plan->exec),
+			"batched step graph");
+	g_plans->push_front(std::move(plan));
+	while(g_plans->size() > MAX_PLANS) {
+		g_plans->pop_back();
+	}
+	return *g_plans->front();
+}
+
+} // namespace
+
+void
+runBatch(Simulation* const sims[], size_t n, const unsigned nsteps[], unsigned long long events[])
+{
+	std::lock_guard<std::mutex> lock(g_planMutex);
+	size_t i = 0;
+	while(i < n) {
+		Simulation* first = sims[i];
+		const unsigned ns0 = nsteps ? nsteps[i] : 0;
+		if(!first->fused() || first->partitions() == 0 || ns0 == 0) {
+			/* plugin-path and empty work run on their own, in order */
+			unsigned long long ev = 0;
+			first->run(ns0, nullptr, &ev);
+			if(events) events[i] = ev;
+			++i;
+			continue;
+		}
+		const int dev = first->device();
+		hipCheck(hipSetDevice(dev), "hipSetDevice");
+		/* greedy pack of fused networks on the same device */
+		uint32_t blockDim = 64, lds = 0, blocks = 0;
+		size_t j = i;
+		std::vector<size_t> members;
+		while(j < n) {
+			Simulation* sim = sims[j];
+			const unsigned ns = nsteps ? nsteps[j] : 0;
+			if(!sim->fused() || sim->device() != dev) break;
+			if(sim->partitions() == 0 || ns == 0) {
+				/* nothing to launch; advance it in order */
+				unsigned long long ev = 0;
+				sim->run(ns, nullptr, &ev);
+				if(events) events[j] = ev;
+				++j;
+				continue;
+			}
+			const uint32_t bd = std::max(blockDim, sim->blockThreads());
+			const uint32_t ld = std::max(lds, sim->ldsBytes());
+			uint32_t maxB = 0;
+			hipCheck(nemo_dev_step_max_blocks(bd, ld, &maxB), "occupancy query");
+			if(blocks + sim->partitions() > maxB) {
+				if(members.empty()) {
+					throw exception(NEMO_API_UNSUPPORTED, "simulation does not fit the device");
+				}
+				break;
+			}
+			blockDim = bd;
+			lds = ld;
+			blocks += sim->partitions();
+			members.push_back(j);
+			++j;
+		}
+		hipStream_t st = batchStream(dev);
+		std::vector<NetDev> nets;
+		std::vector<std::pair<uint32_t, uint32_t> > items;
+		for(size_t k : members) {
+			Simulation* sim = sims[k];
+			const uint32_t idx = uint32_t(nets.size());
+			nets.push_back(sim->prepareLaunch(nsteps[k], st, false));
+			for(uint32_t p = 0; p < sim->partitions(); ++p) items.push_back(std::make_pair(idx, p));
+		}
+		const std::vector<WorkItem> work = Simulation::xcdWorkTable(items);
+		BatchPlan& plan = getPlan(dev, blockDim, lds, uint32_t(nets.size()), work);
+		std::memcpy(plan.hnets, nets.data(), nets.size() * sizeof(NetDev));
+		hipCheck(hipMemcpyAsync(plan.dnets.ptr, plan.hnets, nets.size() * sizeof(NetDev),
+					hipMemcpyHostToDevice, st), "H2D");
+		hipCheck(nemo_dev_graph_launch(plan.exec, st), "batched step graph launch");
+		hipCheck(hipMemcpyAsync(plan.hcollect, plan.dcollect.ptr, nets.size() * 16,
+					hipMemcpyDeviceToHost, st), "D2H");
+		hipCheck(hipStreamSynchronize(st), "hipStreamSynchronize");
+		/* every member ran: advance all of them, then report the first error */
+		bool failed = false;
+		exception firstError(NEMO_OK, "");
+		for(size_t m = 0; m < members.size(); ++m) {
+			const size_t k = members[m];
+			const uint32_t* c = plan.hcollect + 4 * m;
+			if(events) {
+				events[k] = (unsigned long long) c[2] | ((unsigned long long) c[3] << 32);
+			}
+			try {
+				sims[k]->completeLaunch(nsteps[k], c[0]);
+			} catch(exception& e) {
+				if(!failed) firstError = e;
+				failed = true;
+			}
+		}
+		if(failed) throw firstError;
+		i = j;
+	}
+}
+
+} // namespace hip
+} // namespace nemo

@@ -15,6 +15,7 @@
 #include "common.hpp"
 #include "network.hpp"
 #include "simulation.hpp"
+#include "batch.hpp"
 
 using nemo::hip::NetworkImpl;
 using nemo::hip::ConfigurationImpl;
@@ -279,75 +280,12 @@ nemo_hip_run_batch(nemo_simulation_t sims[], size_t n, const unsigned nsteps[], 
 {
 	RETURN_CALL(
 		if(n == 0) return NEMO_OK;
-		for(size_t i = 0; i < n; ++i) requireSim(sims[i]);
-		hipStream_t st;
-		nemo::hipCheck(hipStreamCreate(&st), "hipStreamCreate");
-		struct StreamGuard { hipStream_t s; ~StreamGuard() { (void) hipStreamDestroy(s); } } guard{st};
-		size_t i = 0;
-		nemo::hip::DevBuf dnets, dwork;
-		while(i < n) {
-			/* greedy pack */
-			uint32_t blockDim = 64, lds = 0, blocks = 0;
-			size_t j = i;
-			while(j < n) {
-				Simulation* sim = sims[j]->sim;
-				if(!sim->fused()) {
-					/* plugin-path simulations run on their own, in order */
-					if(j == i) {
-						unsigned long long ev = 0;
-						sim->run(nsteps ? nsteps[j] : 0, nullptr, &ev);
-						if(events) events[j] = ev;
-						++i;
-						++j;
-						continue;
-					}
-					break;
-				}
-				uint32_t bd = std::max(blockDim, sim->blockThreads());
-				uint32_t ld = std::max(lds, sim->ldsBytes());
-				uint32_t maxB = 0;
-				nemo::hipCheck(nemo_dev_step_max_blocks(bd, ld, &maxB), "occupancy query");
-				if(blocks + sim->partitions() > maxB) {
-					if(j == i) throw nemo::exception(NEMO_API_UNSUPPORTED, "simulation does not fit the device");
-					break;
-				}
-				blockDim = bd; lds = ld; blocks += sim->partitions();
-				++j;
-			}
-			std::vector<NetDev> nets;
-			std::vector<std::pair<uint32_t, uint32_t> > items;
-			std::vector<size_t> launched;
-			for(size_t k = i; k < j; ++k) {
-				Simulation* sim = sims[k]->sim;
-				unsigned ns = nsteps ? nsteps[k] : 0;
-				if(sim->partitions() == 0 || ns == 0) {
-					unsigned long long ev = 0;
-					sim->run(ns, nullptr, &ev);
-					if(events) events[k] = ev;
-					continue;
-				}
-				uint32_t idx = uint32_t(nets.size());
-				nets.push_back(sim->prepareLaunch(ns, st));
-				for(uint32_t p = 0; p < sim->partitions(); ++p) items.push_back(std::make_pair(idx, p));
-				launched.push_back(k);
-			}
-			if(!nets.empty()) {
-				std::vector<WorkItem> work = Simulation::xcdWorkTable(items);
-				dnets.ensure(nets.size() * sizeof(NetDev));
-				dwork.ensure(work.size() * sizeof(WorkItem));
-				nemo::hipCheck(hipMemcpyAsync(dnets.ptr, nets.data(), nets.size() * sizeof(NetDev), hipMemcpyHostToDevice, st), "H2D");
-				nemo::hipCheck(hipMemcpyAsync(dwork.ptr, work.data(), work.size() * sizeof(WorkItem), hipMemcpyHostToDevice, st), "H2D");
-				nemo::hipCheck(nemo_dev_step_launch(dnets.as<NetDev>(), dwork.as<WorkItem>(), uint32_t(work.size()),
-							blockDim, lds, st), "batched step kernel launch");
-				nemo::hipCheck(hipStreamSynchronize(st), "hipStreamSynchronize");
-				for(size_t k : launched) {
-					unsigned long long ev = 0;
-					sims[k]->sim->finishLaunch(nsteps[k], st, &ev);
-					if(events) events[k] = ev;
-				}
-			}
-			i = j;
+		std::vector<Simulation*> ss(n);
+		for(size_t i = 0; i < n; ++i) {
+			requireSim(sims[i]);
+			ss[i] = sims[i]->sim;
 		}
+		nemo::hip::runBatch(ss.data(), n, nsteps, events);
 	);
 }
 

@@ -289,6 +289,13 @@ Simulation::Simulation(const NetworkImpl& net, const ConfigurationImpl& conf) :
 		m_words = m_words32 / 2;
 	} else {
 		m_words = m_P * (T / 64u);
+		/* lanes per fired source in delivery: a power of two about 1.5x the
+		 * mean bucket length (synapses of one source into one partition) */
+		const double lavg = m_n ? double(m_nsyn) / (double(m_n) * double(m_P)) : 0.0;
+		m_glog2 = 0;
+		while(m_glog2 < 6 && double(1u << m_glog2) < 1.5 * lavg) {
+			++m_glog2;
+		}
 	}
 
 	/* ---- forward matrix, bucketed by (source, target partition) */
@@ -399,6 +406,10 @@ Simulation::Simulation(const NetworkImpl& net, const ConfigurationImpl& conf) :
 	d_recent.alloc(size_t(m_n) * 8);
 	d_ring.alloc(size_t(m_P) * 2 * m_D * m_T * 4);
 	d_off.upload(off.data(), off.size() * sizeof(uint32_t));
+	if(syn.size() < 2) {
+		/* the delivery loop reads record 0 in place of an absent record */
+		syn.resize(2, 0u);
+	}
 	d_syn.upload(syn.data(), syn.size() * sizeof(uint32_t));
 	d_fl.alloc(size_t(2) * m_P * m_T * 4);
 	d_fc.alloc(size_t(2) * m_P * 8 + 8);
@@ -576,12 +587,14 @@ Simulation::uploadSchedule(unsigned nsteps, const nemo_schedule_t* s)
 }
 
 NetDev
-Simulation::prepareLaunch(unsigned nsteps, void* stream)
+Simulation::prepareLaunch(unsigned nsteps, void* stream, bool clearControl)
 {
 	hipCheck(hipSetDevice(m_device), "hipSetDevice");
 	hipStream_t st = static_cast<hipStream_t>(stream);
 	d_raster.ensure(std::max<size_t>(size_t(nsteps) * m_words, 1) * 8);
-	hipCheck(nemo_dev_fill_u32(d_ctl.as<uint32_t>(), 0, 16, st), "fill");
+	if(clearControl) {
+		hipCheck(nemo_dev_fill_u32(d_ctl.as<uint32_t>(), 0, 16, st), "fill");
+	}
 	NetDev nd;
 	std::memset(&nd, 0, sizeof(nd));
 	nd.n = m_n;
@@ -593,6 +606,7 @@ Simulation::prepareLaunch(unsigned nsteps, void* stream)
 		| (m_replay ? NEMO_NET_REPLAY : 0u);
 	nd.nsteps = nsteps;
 	nd.spinLimit = m_spinLimit;
+	nd.glog2 = m_glog2;
 	nd.cycle0 = m_cycle;
 	nd.model = d_model.as<uint8_t>();
 	nd.param = d_param.as<float>();
@@ -630,18 +644,22 @@ Simulation::finishLaunch(unsigned nsteps, void* stream, unsigned long long* even
 	uint32_t ctl[4];
 	hipCheck(hipMemcpyAsync(ctl, d_ctl.ptr, sizeof(ctl), hipMemcpyDeviceToHost, st), "D2H");
 	hipCheck(hipStreamSynchronize(st), "hipStreamSynchronize");
+	unsigned long long e;
+	std::memcpy(&e, &ctl[2], sizeof(e));
+	completeLaunch(nsteps, ctl[1]);
+	if(events) *events = e;
+}
+
+void
+Simulation::completeLaunch(unsigned nsteps, uint32_t status)
+{
 	m_cycle += nsteps;
 	m_timerCycles += nsteps;
 	m_lastRunSteps = nsteps;
 	m_hasSchedule = false;
 	m_replay = false;
-	if(ctl[1] & NEMO_STATUS_BARRIER_TIMEOUT) {
+	if(status & NEMO_STATUS_BARRIER_TIMEOUT) {
 		throw exception(NEMO_CUDA_ERROR, "step kernel: a workgroup barrier timed out (workgroups not co-resident?)");
-	}
-	if(events) {
-		unsigned long long e;
-		std::memcpy(&e, &ctl[2], sizeof(e));
-		*events = e;
 	}
 }
 

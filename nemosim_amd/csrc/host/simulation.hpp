@@ -96,10 +96,15 @@ class Simulation
 		uint32_t blockThreads() const { return m_T; }
 		uint32_t ldsBytes() const { return m_lds; }
 		/* prepare a launch of nsteps cycles: fills and returns the device
-		 * descriptor (host copy) with per-launch fields set */
-		NetDev prepareLaunch(unsigned nsteps, void* stream);
+		 * descriptor (host copy) with per-launch fields set; clears the
+		 * status word and event counter unless the caller's launch does */
+		NetDev prepareLaunch(unsigned nsteps, void* stream, bool clearControl = true);
+		/* read back status and events after the launch, then completeLaunch */
 		void finishLaunch(unsigned nsteps, void* stream, unsigned long long* events);
+		/* advance the cycle count by a finished launch; throws on a status error */
+		void completeLaunch(unsigned nsteps, uint32_t status);
 		unsigned neuronCount() const { return m_n; }
+		int device() const { return m_device; }
 		/* true: the fused persistent kernel; false: plugin kernels per cycle */
 		bool fused() const { return m_fused; }
 
@@ -132,6 +137,7 @@ class Simulation
 		int m_device = 0;
 		void* m_stream = nullptr;
 		uint32_t m_spinLimit = 1u << 22;
+		uint32_t m_glog2 = 0;
 
 		/* index maps */
 		std::vector<unsigned> m_localToGlobal;

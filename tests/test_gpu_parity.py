@@ -215,6 +215,30 @@ def test_batch_equals_alone(nemo_mod, oracle_mod):
         assert d is None, "network %d: first diverging cycle %d: %s" % ((s.n,) + d)
 
 
+def test_batch_graph_replays(nemo_mod, oracle_mod):
+    """The batched launch is an instantiated HIP graph replayed once per
+    call: repeated calls on the same pack (same graph), a changed pack (a
+    network with 0 cycles drops out, then rejoins) and eager queries between
+    replays; after every replay each network's raster and membrane
+    potentials equal its own oracle run."""
+    sizes = [900, 2100, 1300]
+    specs = [examples.izhikevich_80_20(n, 100, seed=300 + i) for i, n in enumerate(sizes)]
+    sims = [_hip_sim(nemo_mod, s) for s in specs]
+    orcs = [oracle_mod.from_spec(s) for s in specs]
+    chunks = [[50, 50, 50], [50, 50, 50], [30, 70, 50], [60, 0, 40], [50, 50, 50], [50, 50, 50]]
+    for c, ch in enumerate(chunks):
+        nemo_mod.run_batch(sims, ch)
+        for i, (spec, sim, orc, k) in enumerate(zip(specs, sims, orcs, ch)):
+            if k == 0:
+                continue
+            rc, ri = orc.run(k)
+            gc, gi = sim.read_fired()
+            d = first_divergence(rc, ri, gc, gi)
+            assert d is None, "chunk %d network %d: first diverging cycle %d: %s" % ((c, i) + d)
+            for g in (0, spec.n // 2, spec.n - 1):
+                assert _same_float(sim.get_membrane_potential(g), orc.get_membrane_potential(g)), (c, i, g)
+
+
 @pytest.mark.slow
 def test_c2_full_network_prefix(nemo_mod, oracle_mod):
     """The 200k x 1000 network (2e8 synapses) for its first 100 cycles: full
