@@ -87,7 +87,7 @@ def cpu_baseline(spec, target_s, warm_steps, stdp=None):
     }
 
 
-def build_sim(nemosim_amd, spec, stdp=None, partition_size=None, mode="auto"):
+def build_sim(nemosim_amd, spec, stdp=None, partition_size=None, mode="auto", lanes=0):
     net = spec.to_nemo(nemosim_amd)
     conf = nemosim_amd.Configuration()
     conf.set_cuda_backend(0)
@@ -95,6 +95,8 @@ def build_sim(nemosim_amd, spec, stdp=None, partition_size=None, mode="auto"):
     conf.set_step_mode(mode)
     if partition_size:
         conf.set_partition_size(partition_size)
+    if lanes:
+        conf.set_delivery_lanes(lanes)
     if stdp is not None:
         pre, post, a, b, c, d = stdp
         conf.set_stdp_function(pre, post, min_e=a, max_e=b, min_i=c, max_i=d)
@@ -228,7 +230,8 @@ def bench_batch(args):
     specs = [examples.izhikevich_80_20(int(n), 100, seed=args.seed + 1 + i) for i, n in enumerate(sizes)]
     t0 = time.perf_counter()
     plain = [build_sim(nemosim_amd, s) for s in specs]
-    batched = [build_sim(nemosim_amd, s, partition_size=autotune.partition_size(s.n, table)) for s in specs]
+    batched = [build_sim(nemosim_amd, s, partition_size=autotune.partition_size(s.n, table),
+                         lanes=autotune.delivery_lanes(s.n, table)) for s in specs]
     log("  built %d simulations in %.1f s" % (2 * count, time.perf_counter() - t0))
     # plain: one persistent run per network, back to back
     t0 = time.perf_counter()
@@ -335,8 +338,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU; --gpus %d is not supported "
-              "(one network per GPU, see DESIGN.md for how it would shard)" % args.gpus, file=sys.stderr)
+        print("bench.py: this benchmark runs on one GPU; --gpus %d is not supported "
+              "(each network, and each batch of networks, runs on a single GPU)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
     if args.workload == "plain":

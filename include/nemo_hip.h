@@ -48,6 +48,12 @@ nemo_status_t nemo_set_hip_backend(nemo_configuration_t conf, int dev);
  * Configuration::setCudaPartitionSize (src/nemo/Configuration.hpp:61). */
 nemo_status_t nemo_set_partition_size(nemo_configuration_t conf, unsigned neurons);
 
+/* Lanes of a wavefront that share one fired source when its synapses are
+ * delivered (the delivery queue's bin: 1, 2, 4, ..., 64); 0 = automatic
+ * (about 1.5x the mean synapses per source and partition). No counterpart
+ * in NeMo; a tuning knob of the fused step kernel for the autotuner. */
+nemo_status_t nemo_set_delivery_lanes(nemo_configuration_t conf, unsigned lanes);
+
 /* Which step implementation runs a simulation:
  *   NEMO_STEP_AUTO    the fused persistent kernel when every neuron type is a
  *                     built-in model and the network fits it; else plugins
@@ -121,6 +127,9 @@ typedef struct nemo_hip_info {
 	unsigned lds_bytes;
 	unsigned block_threads;
 	float last_kernel_ms;        /* step-kernel time of the last run (HIP events) */
+	unsigned fused;              /* 1: fused persistent step kernel; 0: plugin step */
+	unsigned deferred;           /* 1: records of delay >= 2 delivered one cycle late */
+	unsigned lanes_per_source;   /* lanes sharing one fired source in delivery */
 } nemo_hip_info_t;
 
 nemo_status_t nemo_hip_get_info(nemo_simulation_t sim, nemo_hip_info_t* info);

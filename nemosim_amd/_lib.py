@@ -45,6 +45,9 @@ class InfoT(ctypes.Structure):
         ("lds_bytes", ctypes.c_uint),
         ("block_threads", ctypes.c_uint),
         ("last_kernel_ms", ctypes.c_float),
+        ("fused", ctypes.c_uint),
+        ("deferred", ctypes.c_uint),
+        ("lanes_per_source", ctypes.c_uint),
     ]
 
 
@@ -114,6 +117,7 @@ SIGNATURES = {
                                              c_float_p, c_u8_p]),
     "nemo_set_hip_backend": (ctypes.c_int, [c_void_p, ctypes.c_int]),
     "nemo_set_partition_size": (ctypes.c_int, [c_void_p, ctypes.c_uint]),
+    "nemo_set_delivery_lanes": (ctypes.c_int, [c_void_p, ctypes.c_uint]),
     "nemo_set_step_mode": (ctypes.c_int, [c_void_p, ctypes.c_int]),
     "nemo_hip_run": (ctypes.c_int, [c_void_p, ctypes.c_uint, ctypes.POINTER(ScheduleT), c_ull_p]),
     "nemo_hip_read_fired": (ctypes.c_int, [c_void_p, c_uint_p, c_uint_p, ctypes.c_size_t, c_size_p]),

@@ -18,6 +18,9 @@ extern "C" {
 #define NEMO_NET_SATURATE  2u   /* accumulators may leave the int32 range */
 #define NEMO_NET_REPLAY    4u   /* weights changed since the ring was filled:
                                  * rebuild it from the firing history */
+#define NEMO_NET_DEFER     8u   /* delay-1 records first in each bucket, their
+                                 * count in bits 28..31 of off[]; records of
+                                 * delay >= 2 are delivered one cycle later */
 
 /* Bits of the per-network status word */
 #define NEMO_STATUS_BARRIER_TIMEOUT 1u
@@ -55,7 +58,8 @@ typedef struct NetDev {
 
 	/* per-step exchange between the network's workgroups, double-buffered
 	 * by cycle parity */
-	uint32_t* fl;             /* [2][P][T] fired neurons of each partition */
+	uint64_t* fl;             /* [3][P][T] fired neurons of each partition, (cycle << 32) | neuron
+	                           * (buffer t mod 2, or t mod 3 with NEMO_NET_DEFER) */
 	uint64_t* fc;             /* [2][P] (cycle << 32) | fired count; starts all-ones */
 	uint64_t* rfg;            /* [2][n] firing history (STDP only) */
 	uint32_t* status;         /* NEMO_STATUS_* */

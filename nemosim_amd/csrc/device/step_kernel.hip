@@ -137,6 +137,32 @@ __device__ inline void gather_counts(const NEMO_GLOBAL uint64_t* fc, uint32_t P,
 	if(lane == 0) pref[0] = 0u;
 }
 
+/* A fired-list entry of cycle tag: (tag << 32) | neuron. The count that
+ * names it may become visible first, so a stale entry (of cycle tag-2) is
+ * read again, boundedly, as gather_counts waits. */
+__device__ inline uint32_t fired_entry(const NEMO_GLOBAL uint64_t* e, uint32_t tag, uint32_t spinLimit,
+		uint32_t* status)
+{
+	uint64_t v = ld_agent(e);
+	uint32_t spins = 0;
+	while(uint32_t(v >> 32) != tag) {
+		if(ld_agent(glb(status)) != 0u) {
+			return 0u;
+		}
+		__builtin_amdgcn_s_sleep(1);
+		if(++spins > spinLimit) {
+			atomicOr(status, NEMO_STATUS_BARRIER_TIMEOUT);
+			return 0u;
+		}
+		v = ld_agent(e);
+	}
+	return uint32_t(v);
+}
+
+/* With deferred delivery the bucket start off[s][p] carries, in its top
+ * 4 bits, the number of delay-1 records at the head of the bucket. */
+constexpr uint32_t OFF_NEAR_SHIFT = 28u;
+
 /* Rounds of record loads issued together in the delivery loop */
 constexpr uint32_t DELIVER_RU = 16u;
 
@@ -173,6 +199,68 @@ __device__ inline void ring_add(int32_t* ringE, int32_t* ringI, uint32_t* ovf, u
 	}
 }
 
+/* Delivers the records [a, b) of up to 128 sources held one per lane
+ * (a0/b0 for sources 0..63, a1/b1 for 64..127 of the batch) into the ring,
+ * relative to base slot `base`. Groups of G = 2^glog lanes share one source:
+ * lane i of a group reads records i, i+G, ... of the bucket, so a load
+ * instruction touches a few cache lines instead of 64, and DELIVER_RU rounds
+ * of record loads are issued before their LDS atomics. All lanes of the wave
+ * must call it together (nsrc is wave-uniform). */
+template<bool SAT>
+__device__ inline void deliver_ranges(uint32_t a0, uint32_t b0, uint32_t a1, uint32_t b1, uint32_t nsrc,
+		uint32_t glog, uint32_t lane, uint32_t base, const NEMO_GLOBAL uint2* syn,
+		int32_t* ringE, int32_t* ringI, uint32_t* ovf, uint32_t D, uint32_t T, uint32_t fwords)
+{
+	const uint32_t G = 1u << glog;
+	const uint32_t NG = 64u >> glog;                 /* sources per round */
+	const uint32_t grp = lane >> glog;
+	const uint32_t gl = lane & (G - 1u);
+	const uint32_t rounds = (nsrc + NG - 1u) >> (6u - glog);
+	const bool anyLong = (__ballot((b0 - a0 > G) || (b1 - a1 > G)) != 0ull);
+	for(uint32_t r0 = 0; r0 < rounds; r0 += DELIVER_RU) {    /* wave-uniform */
+		uint2 rec[DELIVER_RU];
+		uint32_t ok = 0;
+#pragma unroll
+		for(uint32_t u = 0; u < DELIVER_RU; ++u) {
+			const uint32_t k = (r0 + u) * NG + grp;      /* source within the batch */
+			const bool hi = ((r0 + u) * NG) >= 64u;     /* wave-uniform */
+			const uint32_t ak = __shfl(hi ? a1 : a0, int(k & 63u), 64);
+			const uint32_t bk = __shfl(hi ? b1 : b0, int(k & 63u), 64);
+			const uint32_t q = ak + gl;
+			const bool valid = (r0 + u < rounds) && (k < nsrc) && (q < bk);
+			/* unconditional load (record 0 stands in for absent ones) so no
+			 * load is left pending on a skipped path */
+			rec[u] = syn[valid ? q : 0u];
+			ok |= uint32_t(valid) << u;
+		}
+#pragma unroll
+		for(uint32_t u = 0; u < DELIVER_RU; ++u) {
+			if((ok >> u) & 1u) {
+				uint32_t sl = base + (rec[u].x >> 16);
+				sl = (sl >= D) ? sl - D : sl;
+				ring_add<SAT>(ringE, ringI, ovf, D, T, fwords, sl, rec[u].x & 0xffffu, int32_t(rec[u].y));
+			}
+		}
+		if(anyLong) {
+			/* records G, 2G, ... of buckets longer than G */
+			for(uint32_t u = 0; u < DELIVER_RU && r0 + u < rounds; ++u) {
+				const uint32_t k = (r0 + u) * NG + grp;
+				const bool hi = ((r0 + u) * NG) >= 64u;
+				const uint32_t ak = __shfl(hi ? a1 : a0, int(k & 63u), 64);
+				const uint32_t bk = __shfl(hi ? b1 : b0, int(k & 63u), 64);
+				if(k < nsrc) {
+					for(uint32_t q = ak + G + gl; q < bk; q += G) {
+						const uint2 r = syn[q];
+						uint32_t sl = base + (r.x >> 16);
+						sl = (sl >= D) ? sl - D : sl;
+						ring_add<SAT>(ringE, ringI, ovf, D, T, fwords, sl, r.x & 0xffffu, int32_t(r.y));
+					}
+				}
+			}
+		}
+	}
+}
+
 template<bool SAT>
 __device__ void run_partition(const NetDev& nd, uint32_t p)
 {
@@ -199,6 +287,10 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 	uint32_t* wsum = pref + ((P + 1u + 63u) & ~63u);
 	uint32_t* ovf = wsum + 64;                                 /* [2][D][T/32] when SAT */
 	const uint32_t fwords = T >> 5;
+	const bool defer = (nd.flags & NEMO_NET_DEFER) != 0u;
+	const uint32_t offMask = defer ? ((1u << OFF_NEAR_SHIFT) - 1u) : ~0u;
+	const NEMO_GLOBAL uint2* syn = glb(reinterpret_cast<const uint2*>(nd.syn));
+	const NEMO_GLOBAL uint32_t* off = glb(nd.off);
 
 	/* restore the pending-current ring */
 	const int32_t* gring = nd.ring + size_t(p) * 2u * D * T;
@@ -220,12 +312,11 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 		__syncthreads();
 		const uint64_t pend = (D >= 64u) ? ~0ull : ((1ull << D) - 1ull);
 		const uint32_t last = uint32_t((nd.cycle0 + D - 1u) % D);    /* slot of cycle0-1 */
-		const NEMO_GLOBAL uint2* syn = glb(reinterpret_cast<const uint2*>(nd.syn));
 		for(uint32_t s = tid; s < n_total; s += nthreads) {
 			const uint64_t h = glb(nd.recent)[s] & pend;
 			if(!h) continue;
-			const NEMO_GLOBAL uint32_t* o = glb(nd.off) + size_t(s) * (P + 1u) + p;
-			const uint32_t a = o[0], b = o[1];
+			const NEMO_GLOBAL uint32_t* o = off + size_t(s) * (P + 1u) + p;
+			const uint32_t a = o[0] & offMask, b = o[1] & offMask;
 			for(uint32_t q = a; q < b; ++q) {
 				const uint2 rec = syn[q];
 				const uint32_t d = rec.x >> 16;
@@ -265,12 +356,18 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 	const StdpParams* sp = reinterpret_cast<const StdpParams*>(nd.stdp);
 	const uint32_t postLen = stdp ? uint32_t(sp->postLen) : 0u;
 	unsigned long long events = 0;
+	/* deferred delivery: this wave's far ranges [a, b) (delays >= 2) of the
+	 * previous cycle's spikes, one source per lane, kn sources (wave-uniform) */
+	uint32_t ka0 = 0, kb0 = 0, ka1 = 0, kb1 = 0, kn = 0;
 	__syncthreads();
 
 	for(uint32_t k = 0; k < nd.nsteps; ++k) {
 		const uint64_t t = nd.cycle0 + k;
 		const uint32_t slot = uint32_t(t % D);
 		const uint32_t par = uint32_t(t & 1u);
+		/* fired-list buffer: by parity, or of three when deferred delivery
+		 * reads the previous cycle's lists while the next are written */
+		const uint32_t flb = defer ? uint32_t(t % 3u) : par;
 
 		/* ---- stimulus of this step (sim-local indices, deduplicated by the host) */
 		bool has_stim = false;
@@ -347,14 +444,28 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 			if(fired) {
 				const uint64_t below = lane ? (mask & (~0ull >> (64u - lane))) : 0ull;
 				const uint32_t pos = base + uint32_t(__popcll(below));
-				st_agent(glb(nd.fl) + (size_t(par) * P + p) * T + pos, n);
+				st_agent(glb(nd.fl) + (size_t(flb) * P + p) * T + pos, (uint64_t(uint32_t(t)) << 32) | uint64_t(n));
 			}
-			/* publish: every wave's fired-list stores complete, then the count */
-			__builtin_amdgcn_s_waitcnt(0);
-			__syncthreads();
+			/* publish the count. Fired-list entries carry the cycle tag, so
+			 * readers validate each entry and need no ordering against the
+			 * count; the STDP firing history does not, so with STDP every
+			 * wave's stores are drained first (s_waitcnt; __syncthreads alone
+			 * waits only for LDS traffic). */
+			if(stdp) {
+				__builtin_amdgcn_s_waitcnt(0);
+				__syncthreads();
+			}
 			if(tid == 0) {
 				st_agent(glb(nd.fc) + par * P + p, (uint64_t(uint32_t(t)) << 32) | uint64_t(total));
 			}
+		}
+
+		/* ---- deferred records (delays >= 2) of the previous cycle's spikes,
+		 * delivered while the other partitions publish this cycle */
+		if(defer && kn) {
+			deliver_ranges<SAT>(ka0, kb0, ka1, kb1, kn, nd.glog2, lane, (slot + D - 1u) % D, syn,
+					ringE, ringI, ovf, D, T, fwords);
+			kn = 0;
 		}
 
 		/* ---- 3. wait for the network's other partitions */
@@ -366,23 +477,11 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 		/* ---- 4. delivery of this cycle's spikes into the ring */
 		{
 			const uint32_t F = pref[P];
-			const NEMO_GLOBAL uint32_t* fl = glb(nd.fl) + size_t(par) * P * T;
-			const NEMO_GLOBAL uint2* syn = glb(reinterpret_cast<const uint2*>(nd.syn));
-			const NEMO_GLOBAL uint32_t* off = glb(nd.off);
+			const NEMO_GLOBAL uint64_t* fl = glb(nd.fl) + size_t(flb) * P * T;
 			/* Each wave takes a contiguous share of the F fired sources and
-			 * first loads, for up to 128 of them at once, the fired-list entry
-			 * and the bucket bounds off[s][p..p+1] (one lane per source, all
-			 * loads in flight together). Then groups of G lanes share one
-			 * source each: lane i of a group reads records i, i+G, ... of the
-			 * bucket, so one load instruction touches a few cache lines
-			 * instead of 64, and up to DELIVER_RU rounds of record loads are
-			 * issued before their LDS atomics. G is the host's choice per
-			 * network (about 1.5x the mean bucket length). */
-			const uint32_t glog = nd.glog2;
-			const uint32_t G = 1u << glog;
-			const uint32_t NG = 64u >> glog;                 /* sources per round */
-			const uint32_t grp = lane >> glog;
-			const uint32_t gl = lane & (G - 1u);
+			 * loads, for up to 128 of them at once, the fired-list entry and
+			 * the bucket bounds off[s][p..p+1] (one lane per source, all loads
+			 * in flight together); deliver_ranges then reads the records. */
 			const uint32_t share = (F + nwaves - 1u) / nwaves;
 			const uint32_t jw0 = min(F, wave * share);
 			const uint32_t jw1 = min(F, jw0 + share);
@@ -397,69 +496,54 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 					const uint32_t q0 = v0 ? partition_of(pref, P, j0) : 0u;
 					const uint32_t q1 = v1 ? partition_of(pref, P, j1) : 0u;
 					uint32_t s0 = 0, s1 = 0;
-					if(v0) s0 = ld_agent(&fl[size_t(q0) * T + (j0 - pref[q0])]);
-					if(v1) s1 = ld_agent(&fl[size_t(q1) * T + (j1 - pref[q1])]);
+					if(v0) s0 = fired_entry(fl + size_t(q0) * T + (j0 - pref[q0]), uint32_t(t), nd.spinLimit, nd.status);
+					if(v1) s1 = fired_entry(fl + size_t(q1) * T + (j1 - pref[q1]), uint32_t(t), nd.spinLimit, nd.status);
 					if(v0) {
-						uint2 ab;
 						const NEMO_GLOBAL uint32_t* o = off + size_t(s0) * (P + 1u) + p;
-						ab.x = o[0];
-						ab.y = o[1];
-						a0 = ab.x;
-						b0 = ab.y;
+						a0 = o[0];
+						b0 = o[1] & offMask;
 					}
 					if(v1) {
-						uint2 ab;
 						const NEMO_GLOBAL uint32_t* o = off + size_t(s1) * (P + 1u) + p;
-						ab.x = o[0];
-						ab.y = o[1];
-						a1 = ab.x;
-						b1 = ab.y;
+						a1 = o[0];
+						b1 = o[1] & offMask;
 					}
 				}
-				events += (b0 - a0) + (b1 - a1);
-				const uint32_t rounds = (nsrc + NG - 1u) >> (6u - glog);
-				const bool anyLong = (__ballot((b0 - a0 > G) || (b1 - a1 > G)) != 0ull);
-				for(uint32_t r0 = 0; r0 < rounds; r0 += DELIVER_RU) {    /* wave-uniform */
-					uint2 rec[DELIVER_RU];
-					uint32_t ok = 0;
-#pragma unroll
-					for(uint32_t u = 0; u < DELIVER_RU; ++u) {
-						const uint32_t k = (r0 + u) * NG + grp;      /* source within the batch */
-						const bool hi = ((r0 + u) * NG) >= 64u;     /* wave-uniform */
-						const uint32_t ak = __shfl(hi ? a1 : a0, int(k & 63u), 64);
-						const uint32_t bk = __shfl(hi ? b1 : b0, int(k & 63u), 64);
-						const uint32_t q = ak + gl;
-						const bool valid = (r0 + u < rounds) && (k < nsrc) && (q < bk);
-						/* unconditional load (record 0 stands in for absent
-						 * ones) so no load is left pending on a skipped path */
-						rec[u] = syn[valid ? q : 0u];
-						ok |= uint32_t(valid) << u;
-					}
-#pragma unroll
-					for(uint32_t u = 0; u < DELIVER_RU; ++u) {
-						if((ok >> u) & 1u) {
-							uint32_t sl = slot + (rec[u].x >> 16);
+				if(defer && jb == jw0) {
+					/* the records of delay 1 (first in each bucket, their
+					 * count in the top bits of the start) now; the rest in
+					 * the next cycle, while the exchange is waited for */
+					const uint32_t n0 = a0 >> OFF_NEAR_SHIFT, n1 = a1 >> OFF_NEAR_SHIFT;
+					a0 &= offMask;
+					a1 &= offMask;
+					events += (b0 - a0) + (b1 - a1);
+					const uint32_t m = max(n0, n1);
+					for(uint32_t i = 0; i < m; ++i) {
+						const bool ok0 = i < n0, ok1 = i < n1;
+						const uint2 r0 = syn[ok0 ? a0 + i : 0u];
+						const uint2 r1 = syn[ok1 ? a1 + i : 0u];
+						if(ok0) {
+							uint32_t sl = slot + (r0.x >> 16);
 							sl = (sl >= D) ? sl - D : sl;
-							ring_add<SAT>(ringE, ringI, ovf, D, T, fwords, sl, rec[u].x & 0xffffu, int32_t(rec[u].y));
+							ring_add<SAT>(ringE, ringI, ovf, D, T, fwords, sl, r0.x & 0xffffu, int32_t(r0.y));
+						}
+						if(ok1) {
+							uint32_t sl = slot + (r1.x >> 16);
+							sl = (sl >= D) ? sl - D : sl;
+							ring_add<SAT>(ringE, ringI, ovf, D, T, fwords, sl, r1.x & 0xffffu, int32_t(r1.y));
 						}
 					}
-					if(anyLong) {
-						/* records G, 2G, ... of buckets longer than G */
-						for(uint32_t u = 0; u < DELIVER_RU && r0 + u < rounds; ++u) {
-							const uint32_t k = (r0 + u) * NG + grp;
-							const bool hi = ((r0 + u) * NG) >= 64u;
-							const uint32_t ak = __shfl(hi ? a1 : a0, int(k & 63u), 64);
-							const uint32_t bk = __shfl(hi ? b1 : b0, int(k & 63u), 64);
-							if(k < nsrc) {
-								for(uint32_t q = ak + G + gl; q < bk; q += G) {
-									const uint2 r = syn[q];
-									uint32_t sl = slot + (r.x >> 16);
-									sl = (sl >= D) ? sl - D : sl;
-									ring_add<SAT>(ringE, ringI, ovf, D, T, fwords, sl, r.x & 0xffffu, int32_t(r.y));
-								}
-							}
-						}
-					}
+					ka0 = a0 + n0;
+					kb0 = b0;
+					ka1 = a1 + n1;
+					kb1 = b1;
+					kn = nsrc;
+				} else {
+					a0 &= offMask;
+					a1 &= offMask;
+					events += (b0 - a0) + (b1 - a1);
+					deliver_ranges<SAT>(a0, b0, a1, b1, nsrc, nd.glog2, lane, slot, syn,
+							ringE, ringI, ovf, D, T, fwords);
 				}
 			}
 		}
@@ -494,6 +578,15 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 					}
 				}
 			}
+		}
+		__syncthreads();
+	}
+
+	/* ---- the last cycle's deferred records */
+	if(defer) {
+		if(kn) {
+			deliver_ranges<SAT>(ka0, kb0, ka1, kb1, kn, nd.glog2, lane,
+					uint32_t((nd.cycle0 + nd.nsteps - 1u) % D), syn, ringE, ringI, ovf, D, T, fwords);
 		}
 		__syncthreads();
 	}

@@ -146,6 +146,17 @@ nemo_status_t nemo_set_write_only_synapses(nemo_configuration_t c) { RETURN_CALL
 nemo_status_t nemo_set_partition_size(nemo_configuration_t c, unsigned n) { RETURN_CALL(c->impl.partitionSize = n); }
 
 nemo_status_t
+nemo_set_delivery_lanes(nemo_configuration_t c, unsigned lanes)
+{
+	RETURN_CALL(
+		if(lanes > 64 || (lanes & (lanes - 1)) != 0) {
+			throw nemo::exception(NEMO_INVALID_INPUT, nemo::fmt("delivery lanes must be 0 or a power of two up to 64, not ", lanes));
+		}
+		c->impl.deliveryLanes = lanes;
+	);
+}
+
+nemo_status_t
 nemo_set_step_mode(nemo_configuration_t c, int mode)
 {
 	RETURN_CALL(

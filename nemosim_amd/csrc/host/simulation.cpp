@@ -293,13 +293,19 @@ Simulation::Simulation(const NetworkImpl& net, const ConfigurationImpl& conf) :
 		 * mean bucket length (synapses of one source into one partition) */
 		const double lavg = m_n ? double(m_nsyn) / (double(m_n) * double(m_P)) : 0.0;
 		m_glog2 = 0;
-		while(m_glog2 < 6 && double(1u << m_glog2) < 1.5 * lavg) {
-			++m_glog2;
+		if(conf.deliveryLanes) {
+			while((1u << m_glog2) < conf.deliveryLanes) ++m_glog2;
+		} else {
+			while(m_glog2 < 6 && double(1u << m_glog2) < 1.5 * lavg) {
+				++m_glog2;
+			}
 		}
 	}
 
 	/* ---- forward matrix, bucketed by (source, target partition) */
 	const uint32_t P1 = m_P + 1;
+	const uint32_t NEAR_SHIFT = 28, NEAR_MAX = 15;
+	std::vector<uint32_t> nearCount;
 	std::vector<uint32_t> off(size_t(m_n) * P1 + 1, 0u);
 	for(size_t i = 0; i < m_nsyn; ++i) {
 		off[size_t(lsrc[i]) * P1 + (m_fused ? ltgt[i] / T : 0u)] += 1;
@@ -316,17 +322,42 @@ Simulation::Simulation(const NetworkImpl& net, const ConfigurationImpl& conf) :
 	std::vector<uint32_t> cursor(off.begin(), off.end());
 	std::vector<uint32_t> syn(2 * m_nsyn);
 	std::vector<uint32_t> fwd(m_nsyn);
-	for(size_t i = 0; i < m_nsyn; ++i) {
-		uint32_t lt = ltgt[i];
-		uint32_t p = m_fused ? lt / T : 0u;
-		uint32_t pos = cursor[size_t(lsrc[i]) * P1 + p]++;
-		if(m_fused) {
-			syn[2 * size_t(pos)] = (lt - p * T) | (uint32_t(Dl[i]) << 16);
-		} else {
-			syn[2 * size_t(pos)] = lt | (uint32_t(Dl[i] - 1) << 26);
+	/* fused step: the delay-1 records of each bucket come first (pass 0),
+	 * so that the kernel can deliver them in the cycle and the rest one
+	 * cycle later */
+	for(int pass = 0; pass < (m_fused ? 2 : 1); ++pass) {
+		for(size_t i = 0; i < m_nsyn; ++i) {
+			if(m_fused && ((Dl[i] == 1) != (pass == 0))) continue;
+			uint32_t lt = ltgt[i];
+			uint32_t p = m_fused ? lt / T : 0u;
+			uint32_t pos = cursor[size_t(lsrc[i]) * P1 + p]++;
+			if(m_fused) {
+				syn[2 * size_t(pos)] = (lt - p * T) | (uint32_t(Dl[i]) << 16);
+			} else {
+				syn[2 * size_t(pos)] = lt | (uint32_t(Dl[i] - 1) << 26);
+			}
+			syn[2 * size_t(pos) + 1] = uint32_t(wfx[i]);
+			fwd[i] = pos;
 		}
-		syn[2 * size_t(pos) + 1] = uint32_t(wfx[i]);
-		fwd[i] = pos;
+		if(m_fused && pass == 0) {
+			/* cursor - off = delay-1 records per bucket; the deferred form
+			 * keeps that count in the top bits of the bucket start */
+			m_defer = run < (uint64_t(1) << NEAR_SHIFT);
+			for(size_t k = 0; k + 1 < off.size() && m_defer; ++k) {
+				if(cursor[k] - off[k] > NEAR_MAX) m_defer = false;
+			}
+			if(m_defer) {
+				for(size_t k = 0; k + 1 < off.size(); ++k) {
+					nearCount.push_back(cursor[k] - off[k]);
+				}
+			}
+		}
+	}
+	if(m_defer) {
+		for(size_t k = 0; k < nearCount.size(); ++k) {
+			off[k] |= nearCount[k] << NEAR_SHIFT;
+		}
+		std::vector<uint32_t>().swap(nearCount);
 	}
 	if(!m_fused) {
 		/* rows per source: off[s] .. off[s+1] */
@@ -411,7 +442,7 @@ Simulation::Simulation(const NetworkImpl& net, const ConfigurationImpl& conf) :
 		syn.resize(2, 0u);
 	}
 	d_syn.upload(syn.data(), syn.size() * sizeof(uint32_t));
-	d_fl.alloc(size_t(2) * m_P * m_T * 4);
+	d_fl.alloc(size_t(3) * m_P * m_T * 8);
 	d_fc.alloc(size_t(2) * m_P * 8 + 8);
 	d_ctl.alloc(64);
 	if(m_stdp) {
@@ -429,6 +460,8 @@ Simulation::Simulation(const NetworkImpl& net, const ConfigurationImpl& conf) :
 	if(d_rcmAcc.bytes) hipCheck(nemo_dev_fill_u32(d_rcmAcc.as<uint32_t>(), 0, d_rcmAcc.bytes / 4, s), "fill");
 	/* tagged fired counts: all-ones never matches a cycle's tag */
 	if(d_fc.bytes) hipCheck(nemo_dev_fill_u64(d_fc.as<uint64_t>(), ~0ull, d_fc.bytes / 8, s), "fill");
+	/* fired-list entries are tagged with their cycle: no entry is valid yet */
+	if(d_fl.bytes) hipCheck(nemo_dev_fill_u64(d_fl.as<uint64_t>(), ~0ull, d_fl.bytes / 8, s), "fill");
 	hipCheck(nemo_dev_fill_u32(d_ctl.as<uint32_t>(), 0, 16, s), "fill");
 
 	if(m_fused) {
@@ -603,7 +636,7 @@ Simulation::prepareLaunch(unsigned nsteps, void* stream, bool clearControl)
 	nd.D = m_D;
 	nd.words = m_words;
 	nd.flags = (m_stdp ? NEMO_NET_STDP : 0u) | (m_saturate ? NEMO_NET_SATURATE : 0u)
-		| (m_replay ? NEMO_NET_REPLAY : 0u);
+		| (m_replay ? NEMO_NET_REPLAY : 0u) | (m_defer ? NEMO_NET_DEFER : 0u);
 	nd.nsteps = nsteps;
 	nd.spinLimit = m_spinLimit;
 	nd.glog2 = m_glog2;
@@ -616,7 +649,7 @@ Simulation::prepareLaunch(unsigned nsteps, void* stream, bool clearControl)
 	nd.ring = d_ring.as<int32_t>();
 	nd.off = d_off.as<uint32_t>();
 	nd.syn = d_syn.as<uint32_t>();
-	nd.fl = d_fl.as<uint32_t>();
+	nd.fl = d_fl.as<uint64_t>();
 	nd.fc = d_fc.as<uint64_t>();
 	nd.rfg = d_rfg.as<uint64_t>();
 	nd.status = d_ctl.as<uint32_t>() + 1;
@@ -880,6 +913,9 @@ Simulation::info(nemo_hip_info_t* out) const
 	out->lds_bytes = m_lds;
 	out->block_threads = m_T;
 	out->last_kernel_ms = m_kernelMs;
+	out->fused = m_fused ? 1u : 0u;
+	out->deferred = m_defer ? 1u : 0u;
+	out->lanes_per_source = m_fused ? (1u << m_glog2) : 0u;
 }
 
 /* src/nemo/ConnectivityMatrix.cpp:241-294 */

@@ -27,6 +27,7 @@ struct ConfigurationImpl {
 	bool writeOnlySynapses = false;
 	bool logging = false;
 	unsigned partitionSize = 0;   /* 0 = automatic */
+	unsigned deliveryLanes = 0;   /* lanes per fired source in delivery; 0 = automatic */
 	int stepMode = NEMO_STEP_AUTO;
 	std::string description;
 
@@ -138,6 +139,8 @@ class Simulation
 		void* m_stream = nullptr;
 		uint32_t m_spinLimit = 1u << 22;
 		uint32_t m_glog2 = 0;
+		/* fused step delivers delay >= 2 records one cycle late (NEMO_NET_DEFER) */
+		bool m_defer = false;
 
 		/* index maps */
 		std::vector<unsigned> m_localToGlobal;

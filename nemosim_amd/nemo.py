@@ -92,6 +92,10 @@ class Configuration(object):
     def set_partition_size(self, neurons):
         _check(self._lib.nemo_set_partition_size(self._h, int(neurons)))
 
+    def set_delivery_lanes(self, lanes):
+        """Lanes sharing one fired source in delivery (power of two <= 64; 0 = automatic)."""
+        _check(self._lib.nemo_set_delivery_lanes(self._h, int(lanes)))
+
     def set_step_mode(self, mode):
         """'auto', 'fused' or 'plugins' (see nemo_set_step_mode)."""
         _check(self._lib.nemo_set_step_mode(self._h, STEP_MODES[mode]))

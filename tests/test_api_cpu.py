@@ -117,3 +117,13 @@ def test_stdp_window_too_long(nemo_mod):
     conf = nemo_mod.Configuration()
     with pytest.raises(nemo_mod.NemoError):
         conf.set_stdp_function([0.1] * 40, [-0.1] * 30, min_e=0, max_e=1, min_i=0, max_i=-1)
+
+
+def test_delivery_lanes_validation(nemo_mod):
+    conf = nemo_mod.Configuration()
+    for ok in (0, 1, 8, 64):
+        conf.set_delivery_lanes(ok)
+    for bad in (3, 12, 128):
+        with pytest.raises(nemo_mod.NemoError) as e:
+            conf.set_delivery_lanes(bad)
+        assert e.value.status == 6   # NEMO_INVALID_INPUT

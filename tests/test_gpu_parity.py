@@ -215,6 +215,30 @@ def test_batch_equals_alone(nemo_mod, oracle_mod):
         assert d is None, "network %d: first diverging cycle %d: %s" % ((s.n,) + d)
 
 
+@pytest.mark.parametrize("dense", [False, True])
+def test_deferred_and_immediate_delivery(nemo_mod, oracle_mod, dense):
+    """The fused kernel delivers delay-1 records in the cycle and the rest one
+    cycle later when every (source, partition) bucket has at most 15 delay-1
+    records; a denser bucket switches the network to immediate delivery of
+    all delays. Both forms give the oracle's raster and state."""
+    from backends import HipBuilder, OracleBuilder
+    spec = examples.izhikevich_80_20(1500, 100, seed=77)
+    builders = [HipBuilder(nemo_mod), OracleBuilder(oracle_mod)]
+    for b in builders:
+        b.load_spec(spec)
+        if dense:
+            for s in range(20):
+                for t in range(40):
+                    b.add_synapse(s, t, 1, 0.3, 0)
+    sim = builders[0].simulation()
+    orc = builders[1].simulation()
+    info = sim.info()
+    assert info["fused"] == 1
+    assert info["deferred"] == (0 if dense else 1), info
+    _compare_run(sim, orc, 400)
+    _compare_state(sim, orc, spec.n, 2, sample=range(0, spec.n, 7))
+
+
 def test_batch_graph_replays(nemo_mod, oracle_mod):
     """The batched launch is an instantiated HIP graph replayed once per
     call: repeated calls on the same pack (same graph), a changed pack (a
@@ -247,3 +271,29 @@ def test_c2_full_network_prefix(nemo_mod, oracle_mod):
     sim = _hip_sim(nemo_mod, spec)
     orc = oracle_mod.from_spec(spec)
     _compare_run(sim, orc, 100)
+
+
+@pytest.mark.slow
+def test_c3_stdp_epochs(nemo_mod, oracle_mod):
+    """The 50k x 1000 STDP network (excitatory synapses plastic, weights in
+    [0,1]) for three 1000-cycle epochs with applyStdp(1.0) after each:
+    rasters bit-exact every epoch, then a sample of 20000 weights."""
+    n = 50000
+    spec = examples.izhikevich_80_20(n, 1000, seed=3, stdp=True)
+    pre, post = examples.stdp_window()
+    stdp = (pre, post, 0.0, 1.0, 0.0, -1.0)
+    sim = _hip_sim(nemo_mod, spec, stdp=stdp)
+    orc = oracle_mod.from_spec(spec, stdp=stdp)
+    for epoch in range(3):
+        _compare_run(sim, orc, 1000)
+        sim.apply_stdp(1.0)
+        orc.apply_stdp(1.0)
+    ref_w = orc.weights_fx(n, spec.nsyn)
+    sample = np.random.default_rng(1).integers(0, spec.nsyn, size=20000)
+    for k in sample:
+        src = int(np.searchsorted(spec.rowptr, k, side="right") - 1)
+        sid = (src << 32) | int(k - spec.rowptr[src])
+        got = sim.get_synapse_weight(sid)
+        exp = np.float32(np.float32(ref_w[k]) / np.float32(1 << 20))
+        assert got == exp, (k, got, exp)
+
