@@ -92,8 +92,7 @@ NEMO_HD inline float logf_glibc(float x)
 	return float(y);
 }
 
-/* Sine / cosine polynomial set; the second set negates the cosine
- * coefficients for quadrants 2 and 3. */
+/* Sine / cosine polynomial coefficients */
 struct SincosfSet {
 	double c0, c1, c2, c3, c4, s1, s2, s3;
 };
@@ -117,14 +116,15 @@ NEMO_HD inline float sinf_poly(double x, double x2, const SincosfSet& p, int n)
 }
 
 /* sinf for a in [0, 2*pi]: |a| < pi/4 directly, otherwise one-step
- * reduction by pi/2 (rounded through a 2^24-scaled integer). */
+ * reduction by pi/2 (rounded through a 2^24-scaled integer). In quadrants
+ * 2 and 3 glibc evaluates the cosine polynomial with every coefficient
+ * negated; negation is exact in each product and sum, so that equals the
+ * negated result of the positive polynomial, which is what is computed
+ * here (one coefficient set, no per-lane selection of constants). */
 NEMO_HD inline float sinf_glibc(float y)
 {
 	const SincosfSet P0 = { 0x1p0, -0x1.ffffffd0c621cp-2, 0x1.55553e1068f19p-5,
 		-0x1.6c087e89a359dp-10, 0x1.99343027bf8c3p-16,
-		-0x1.555545995a603p-3, 0x1.1107605230bc4p-7, -0x1.994eb3774cf24p-13 };
-	const SincosfSet P1 = { -0x1p0, 0x1.ffffffd0c621cp-2, -0x1.55553e1068f19p-5,
-		0x1.6c087e89a359dp-10, -0x1.99343027bf8c3p-16,
 		-0x1.555545995a603p-3, 0x1.1107605230bc4p-7, -0x1.994eb3774cf24p-13 };
 	const double HPI_INV = 0x1.45F306DC9C883p+23;
 	const double HPI = 0x1.921FB54442D18p0;
@@ -141,8 +141,8 @@ NEMO_HD inline float sinf_glibc(float y)
 	int n = (int32_t(r) + 0x800000) >> 24;
 	x = x - double(n) * HPI;
 	double s = ((n & 3) == 1 || (n & 3) == 2) ? -1.0 : 1.0;
-	const SincosfSet& p = (n & 2) ? P1 : P0;
-	return sinf_poly(x * s, x * x, p, n);
+	const float v = sinf_poly(x * s, x * x, P0, n);
+	return ((n & 3) == 3) ? -v : v;
 }
 
 } // namespace nemo_libm

@@ -18,6 +18,7 @@ extern "C" {
 #define NEMO_NET_SATURATE  2u   /* accumulators may leave the int32 range */
 #define NEMO_NET_REPLAY    4u   /* weights changed since the ring was filled:
                                  * rebuild it from the firing history */
+#define NEMO_NET_IZHIKEVICH_ONLY 16u  /* every neuron is an Izhikevich neuron */
 #define NEMO_NET_DEFER     8u   /* delay-1 records first in each bucket, their
                                  * count in bits 28..31 of off[]; records of
                                  * delay >= 2 are delivered one cycle later */
@@ -90,14 +91,22 @@ typedef struct WorkItem {
 
 /* Kernel launchers (nemosim_amd/csrc/device/step_kernel.hip). They return a
  * hipError_t value as int and take a hipStream_t as void*. */
+/* Step-kernel variants: bit 0 = saturating accumulation (NEMO_NET_SATURATE),
+ * bit 1 = Izhikevich-only network (NEMO_NET_IZHIKEVICH_ONLY). All networks
+ * of one launch must be of the launch's variant. */
+static inline uint32_t nemo_step_variant(uint32_t flags)
+{
+	return ((flags & NEMO_NET_SATURATE) ? 1u : 0u) | ((flags & NEMO_NET_IZHIKEVICH_ONLY) ? 2u : 0u);
+}
 int nemo_dev_step_launch(const NetDev* nets_dev, const WorkItem* work_dev, uint32_t nblocks,
-		uint32_t blockDim, uint32_t ldsBytes, void* stream);
-int nemo_dev_step_max_blocks(uint32_t blockDim, uint32_t ldsBytes, uint32_t* maxBlocks);
+		uint32_t blockDim, uint32_t ldsBytes, uint32_t variant, void* stream);
+int nemo_dev_step_max_blocks(uint32_t blockDim, uint32_t ldsBytes, uint32_t variant, uint32_t* maxBlocks);
 /* Batched step as an instantiated HIP graph (clear -> cooperative step ->
  * collect); collect_dev receives [nnets][4] words {status, 0, events lo, hi}.
  * The graph reads nets_dev/work_dev at replay time. */
 int nemo_dev_batch_graph_create(const NetDev* nets_dev, uint32_t nnets, const WorkItem* work_dev,
-		uint32_t nblocks, uint32_t blockDim, uint32_t ldsBytes, uint32_t* collect_dev, void** exec_out);
+		uint32_t nblocks, uint32_t blockDim, uint32_t ldsBytes, uint32_t variant, uint32_t* collect_dev,
+		void** exec_out);
 int nemo_dev_graph_launch(void* exec, void* stream);
 int nemo_dev_graph_destroy(void* exec);
 int nemo_dev_apply_stdp(uint32_t* syn, const uint32_t* rcm_fwd, int32_t* rcm_acc, size_t nr,

@@ -98,41 +98,59 @@ __device__ inline V ld_agent(const NEMO_GLOBAL V* p)
  * fired count for cycle t (count tagged with the low 32 bits of t; the
  * buffer of the same parity last held cycle t-2), then writes the exclusive
  * prefix of the counts to pref[0..P]. This is the network's only grid-level
- * synchronisation per cycle. The wait is bounded: after spinLimit polls the
- * status word is flagged, and a flagged status makes every later wait of the
- * launch fall through, so all waves run to completion. */
+ * synchronisation per cycle. Each lane polls up to GATHER_CH partitions
+ * (lane, lane+64, ...) with all their loads in flight together, and polls
+ * again only those not yet published. The wait is bounded: after spinLimit
+ * polls the status word is flagged, and a flagged status makes every later
+ * wait of the launch fall through, so all waves run to completion. */
+constexpr uint32_t GATHER_CH = 4u;
+
 __device__ inline void gather_counts(const NEMO_GLOBAL uint64_t* fc, uint32_t P, uint32_t tag, uint32_t* pref,
 		uint32_t lane, uint32_t spinLimit, uint32_t* status)
 {
 	uint32_t carry = 0;
-	for(uint32_t c = 0; c < P; c += 64u) {
-		uint32_t cnt = 0;
-		if(c + lane < P) {
-			uint32_t spins = 0;
-			for(;;) {
-				const uint64_t v = ld_agent(&fc[c + lane]);
-				if(uint32_t(v >> 32) == tag) {
-					cnt = uint32_t(v);
-					break;
-				}
-				if(ld_agent(glb(status)) != 0u) {
-					break;
-				}
-				__builtin_amdgcn_s_sleep(1);
-				if(++spins > spinLimit) {
-					atomicOr(status, NEMO_STATUS_BARRIER_TIMEOUT);
-					break;
+	for(uint32_t c0 = 0; c0 < P; c0 += 64u * GATHER_CH) {
+		uint32_t cnt[GATHER_CH];
+		uint32_t pending = 0;
+#pragma unroll
+		for(uint32_t i = 0; i < GATHER_CH; ++i) {
+			cnt[i] = 0u;
+			if(c0 + 64u * i + lane < P) pending |= 1u << i;
+		}
+		uint32_t spins = 0;
+		while(pending) {
+			uint64_t v[GATHER_CH];
+#pragma unroll
+			for(uint32_t i = 0; i < GATHER_CH; ++i) {
+				v[i] = ((pending >> i) & 1u) ? ld_agent(fc + c0 + 64u * i + lane) : 0ull;
+			}
+#pragma unroll
+			for(uint32_t i = 0; i < GATHER_CH; ++i) {
+				if(((pending >> i) & 1u) && uint32_t(v[i] >> 32) == tag) {
+					cnt[i] = uint32_t(v[i]);
+					pending &= ~(1u << i);
 				}
 			}
+			if(!pending) break;
+			if(ld_agent(glb(status)) != 0u) break;
+			__builtin_amdgcn_s_sleep(1);
+			if(++spins > spinLimit) {
+				atomicOr(status, NEMO_STATUS_BARRIER_TIMEOUT);
+				break;
+			}
 		}
-		uint32_t x = cnt;
 #pragma unroll
-		for(uint32_t d = 1; d < 64u; d <<= 1) {
-			uint32_t y = __shfl_up(x, d, 64);
-			if(lane >= d) x += y;
+		for(uint32_t i = 0; i < GATHER_CH; ++i) {
+			uint32_t x = cnt[i];
+#pragma unroll
+			for(uint32_t d = 1; d < 64u; d <<= 1) {
+				uint32_t y = __shfl_up(x, d, 64);
+				if(lane >= d) x += y;
+			}
+			const uint32_t q = c0 + 64u * i + lane;
+			if(q < P) pref[q + 1] = carry + x;
+			carry += __shfl(x, 63, 64);
 		}
-		if(c + lane < P) pref[c + lane + 1] = carry + x;
-		carry += __shfl(x, 63, 64);
 	}
 	if(lane == 0) pref[0] = 0u;
 }
@@ -159,12 +177,25 @@ __device__ inline uint32_t fired_entry(const NEMO_GLOBAL uint64_t* e, uint32_t t
 	return uint32_t(v);
 }
 
+/* Loads of a synapse record and of a bucket bound in the delivery loops */
+__device__ inline uint2 ld_rec(const NEMO_GLOBAL uint2* syn, uint32_t q)
+{
+	return syn[q];
+}
+
+__device__ inline uint32_t ld_off(const NEMO_GLOBAL uint32_t* o)
+{
+	return *o;
+}
+
 /* With deferred delivery the bucket start off[s][p] carries, in its top
  * 4 bits, the number of delay-1 records at the head of the bucket. */
 constexpr uint32_t OFF_NEAR_SHIFT = 28u;
 
-/* Rounds of record loads issued together in the delivery loop */
-constexpr uint32_t DELIVER_RU = 16u;
+/* Delivery rounds whose record loads are issued together, and records per
+ * lane per round (lane i of a group reads records i and i+G) */
+constexpr uint32_t DELIVER_RU = 8u;
+constexpr uint32_t DELIVER_KPL = 2u;
 
 /* Partition whose fired list holds flat index j: the q with
  * pref[q] <= j < pref[q+1] (pref is the exclusive prefix of counts) */
@@ -202,10 +233,11 @@ __device__ inline void ring_add(int32_t* ringE, int32_t* ringI, uint32_t* ovf, u
 /* Delivers the records [a, b) of up to 128 sources held one per lane
  * (a0/b0 for sources 0..63, a1/b1 for 64..127 of the batch) into the ring,
  * relative to base slot `base`. Groups of G = 2^glog lanes share one source:
- * lane i of a group reads records i, i+G, ... of the bucket, so a load
+ * lane i of a group reads records i and i+G of the bucket, so a load
  * instruction touches a few cache lines instead of 64, and DELIVER_RU rounds
- * of record loads are issued before their LDS atomics. All lanes of the wave
- * must call it together (nsrc is wave-uniform). */
+ * of record loads are issued before their LDS atomics. Records from 2G on
+ * (rare when G is about 1.5x the mean bucket) are read one per lane per
+ * step. All lanes of the wave must call it together (nsrc is wave-uniform). */
 template<bool SAT>
 __device__ inline void deliver_ranges(uint32_t a0, uint32_t b0, uint32_t a1, uint32_t b1, uint32_t nsrc,
 		uint32_t glog, uint32_t lane, uint32_t base, const NEMO_GLOBAL uint2* syn,
@@ -216,9 +248,10 @@ __device__ inline void deliver_ranges(uint32_t a0, uint32_t b0, uint32_t a1, uin
 	const uint32_t grp = lane >> glog;
 	const uint32_t gl = lane & (G - 1u);
 	const uint32_t rounds = (nsrc + NG - 1u) >> (6u - glog);
-	const bool anyLong = (__ballot((b0 - a0 > G) || (b1 - a1 > G)) != 0ull);
+	const uint32_t covered = DELIVER_KPL * G;        /* records per bucket taken by the rounds */
+	const bool anyLong = (__ballot((b0 - a0 > covered) || (b1 - a1 > covered)) != 0ull);
 	for(uint32_t r0 = 0; r0 < rounds; r0 += DELIVER_RU) {    /* wave-uniform */
-		uint2 rec[DELIVER_RU];
+		uint2 rec[DELIVER_RU][DELIVER_KPL];
 		uint32_t ok = 0;
 #pragma unroll
 		for(uint32_t u = 0; u < DELIVER_RU; ++u) {
@@ -226,31 +259,39 @@ __device__ inline void deliver_ranges(uint32_t a0, uint32_t b0, uint32_t a1, uin
 			const bool hi = ((r0 + u) * NG) >= 64u;     /* wave-uniform */
 			const uint32_t ak = __shfl(hi ? a1 : a0, int(k & 63u), 64);
 			const uint32_t bk = __shfl(hi ? b1 : b0, int(k & 63u), 64);
-			const uint32_t q = ak + gl;
-			const bool valid = (r0 + u < rounds) && (k < nsrc) && (q < bk);
-			/* unconditional load (record 0 stands in for absent ones) so no
-			 * load is left pending on a skipped path */
-			rec[u] = syn[valid ? q : 0u];
-			ok |= uint32_t(valid) << u;
+			const bool live = (r0 + u < rounds) && (k < nsrc);
+#pragma unroll
+			for(uint32_t i = 0; i < DELIVER_KPL; ++i) {
+				const uint32_t q = ak + gl + i * G;
+				const bool valid = live && (q < bk);
+				/* unconditional load (record 0 stands in for absent ones) so
+				 * no load is left pending on a skipped path */
+				rec[u][i] = ld_rec(syn, valid ? q : 0u);
+				ok |= uint32_t(valid) << (u * DELIVER_KPL + i);
+			}
 		}
 #pragma unroll
 		for(uint32_t u = 0; u < DELIVER_RU; ++u) {
-			if((ok >> u) & 1u) {
-				uint32_t sl = base + (rec[u].x >> 16);
-				sl = (sl >= D) ? sl - D : sl;
-				ring_add<SAT>(ringE, ringI, ovf, D, T, fwords, sl, rec[u].x & 0xffffu, int32_t(rec[u].y));
+#pragma unroll
+			for(uint32_t i = 0; i < DELIVER_KPL; ++i) {
+				if((ok >> (u * DELIVER_KPL + i)) & 1u) {
+					const uint2 r = rec[u][i];
+					uint32_t sl = base + (r.x >> 16);
+					sl = (sl >= D) ? sl - D : sl;
+					ring_add<SAT>(ringE, ringI, ovf, D, T, fwords, sl, r.x & 0xffffu, int32_t(r.y));
+				}
 			}
 		}
 		if(anyLong) {
-			/* records G, 2G, ... of buckets longer than G */
+			/* records 2G, 2G+G, ... of buckets longer than 2G */
 			for(uint32_t u = 0; u < DELIVER_RU && r0 + u < rounds; ++u) {
 				const uint32_t k = (r0 + u) * NG + grp;
 				const bool hi = ((r0 + u) * NG) >= 64u;
 				const uint32_t ak = __shfl(hi ? a1 : a0, int(k & 63u), 64);
 				const uint32_t bk = __shfl(hi ? b1 : b0, int(k & 63u), 64);
 				if(k < nsrc) {
-					for(uint32_t q = ak + G + gl; q < bk; q += G) {
-						const uint2 r = syn[q];
+					for(uint32_t q = ak + covered + gl; q < bk; q += G) {
+						const uint2 r = ld_rec(syn, q);
 						uint32_t sl = base + (r.x >> 16);
 						sl = (sl >= D) ? sl - D : sl;
 						ring_add<SAT>(ringE, ringI, ovf, D, T, fwords, sl, r.x & 0xffffu, int32_t(r.y));
@@ -261,9 +302,13 @@ __device__ inline void deliver_ranges(uint32_t a0, uint32_t b0, uint32_t a1, uin
 	}
 }
 
-template<bool SAT>
+/* IZO: every neuron of the network is an Izhikevich neuron, so only its 5
+ * parameters and 2 state variables are held in registers. */
+template<bool SAT, bool IZO>
 __device__ void run_partition(const NetDev& nd, uint32_t p)
 {
+	constexpr int NP = IZO ? 5 : MAX_PARAM;
+	constexpr int NS = IZO ? 2 : MAX_STATE;
 	extern __shared__ __align__(16) uint8_t lds[];
 
 	const uint32_t T = nd.T;
@@ -272,7 +317,7 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 	const uint32_t n_total = nd.n;
 	const uint32_t tid = threadIdx.x;
 	const uint32_t nthreads = blockDim.x;
-	const uint32_t wave = tid >> 6;
+	const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);   /* wave-uniform, in a scalar register */
 	const uint32_t lane = tid & 63u;
 	const uint32_t nwaves = nthreads >> 6;
 	const uint32_t n = p * T + tid;
@@ -332,21 +377,21 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 
 	/* neuron registers */
 	int model = 0;
-	float prm[MAX_PARAM];
-	float st[MAX_STATE];
+	float prm[NP];
+	float st[NS];
 	Rng rng;
 	uint64_t recent = 0;
 #pragma unroll
-	for(int k = 0; k < MAX_PARAM; ++k) prm[k] = 0.0f;
+	for(int k = 0; k < NP; ++k) prm[k] = 0.0f;
 #pragma unroll
-	for(int k = 0; k < MAX_STATE; ++k) st[k] = 0.0f;
+	for(int k = 0; k < NS; ++k) st[k] = 0.0f;
 	rng.s[0] = rng.s[1] = rng.s[2] = rng.s[3] = 0u;
 	if(active) {
 		model = nd.model[n];
 #pragma unroll
-		for(int k = 0; k < MAX_PARAM; ++k) prm[k] = nd.param[size_t(k) * n_total + n];
+		for(int k = 0; k < NP; ++k) prm[k] = nd.param[size_t(k) * n_total + n];
 #pragma unroll
-		for(int k = 0; k < MAX_STATE; ++k) st[k] = nd.state[size_t(k) * n_total + n];
+		for(int k = 0; k < NS; ++k) st[k] = nd.state[size_t(k) * n_total + n];
 #pragma unroll
 		for(int k = 0; k < 4; ++k) rng.s[k] = nd.rng[size_t(k) * n_total + n];
 		recent = nd.recent[n];
@@ -415,7 +460,11 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 				cx = ext[tid];
 				fs = ((fst[tid >> 5] >> (tid & 31u)) & 1u) != 0u;
 			}
-			fired = update_neuron(model, prm, st, rng, fx20_to_float(e), fx20_to_float(i), cx, fs);
+			if(IZO) {
+				fired = izhikevich(prm, st, rng, fx20_to_float(e), fx20_to_float(i), cx, fs);
+			} else {
+				fired = update_neuron(model, prm, st, rng, fx20_to_float(e), fx20_to_float(i), cx, fs);
+			}
 			recent = (recent << 1) | (fired ? 1ull : 0ull);
 			if(stdp) {
 				st_agent(glb(nd.rfg) + size_t(par) * n_total + n, recent);
@@ -500,13 +549,13 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 					if(v1) s1 = fired_entry(fl + size_t(q1) * T + (j1 - pref[q1]), uint32_t(t), nd.spinLimit, nd.status);
 					if(v0) {
 						const NEMO_GLOBAL uint32_t* o = off + size_t(s0) * (P + 1u) + p;
-						a0 = o[0];
-						b0 = o[1] & offMask;
+						a0 = ld_off(o);
+						b0 = ld_off(o + 1) & offMask;
 					}
 					if(v1) {
 						const NEMO_GLOBAL uint32_t* o = off + size_t(s1) * (P + 1u) + p;
-						a1 = o[0];
-						b1 = o[1] & offMask;
+						a1 = ld_off(o);
+						b1 = ld_off(o + 1) & offMask;
 					}
 				}
 				if(defer && jb == jw0) {
@@ -517,21 +566,19 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 					a0 &= offMask;
 					a1 &= offMask;
 					events += (b0 - a0) + (b1 - a1);
+					/* delay-1 records: slot t+1; two per source loaded together */
 					const uint32_t m = max(n0, n1);
-					for(uint32_t i = 0; i < m; ++i) {
-						const bool ok0 = i < n0, ok1 = i < n1;
-						const uint2 r0 = syn[ok0 ? a0 + i : 0u];
-						const uint2 r1 = syn[ok1 ? a1 + i : 0u];
-						if(ok0) {
-							uint32_t sl = slot + (r0.x >> 16);
-							sl = (sl >= D) ? sl - D : sl;
-							ring_add<SAT>(ringE, ringI, ovf, D, T, fwords, sl, r0.x & 0xffffu, int32_t(r0.y));
-						}
-						if(ok1) {
-							uint32_t sl = slot + (r1.x >> 16);
-							sl = (sl >= D) ? sl - D : sl;
-							ring_add<SAT>(ringE, ringI, ovf, D, T, fwords, sl, r1.x & 0xffffu, int32_t(r1.y));
-						}
+					for(uint32_t i = 0; i < m; i += 2u) {
+						const bool ok00 = i < n0, ok01 = i + 1u < n0, ok10 = i < n1, ok11 = i + 1u < n1;
+						const uint2 r00 = ld_rec(syn, ok00 ? a0 + i : 0u);
+						const uint2 r01 = ld_rec(syn, ok01 ? a0 + i + 1u : 0u);
+						const uint2 r10 = ld_rec(syn, ok10 ? a1 + i : 0u);
+						const uint2 r11 = ld_rec(syn, ok11 ? a1 + i + 1u : 0u);
+						const uint32_t sl = (slot + 1u >= D) ? slot + 1u - D : slot + 1u;
+						if(ok00) ring_add<SAT>(ringE, ringI, ovf, D, T, fwords, sl, r00.x & 0xffffu, int32_t(r00.y));
+						if(ok01) ring_add<SAT>(ringE, ringI, ovf, D, T, fwords, sl, r01.x & 0xffffu, int32_t(r01.y));
+						if(ok10) ring_add<SAT>(ringE, ringI, ovf, D, T, fwords, sl, r10.x & 0xffffu, int32_t(r10.y));
+						if(ok11) ring_add<SAT>(ringE, ringI, ovf, D, T, fwords, sl, r11.x & 0xffffu, int32_t(r11.y));
 					}
 					ka0 = a0 + n0;
 					kb0 = b0;
@@ -594,7 +641,7 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 	/* ---- save state and the ring for the next launch */
 	if(active) {
 #pragma unroll
-		for(int k = 0; k < MAX_STATE; ++k) nd.state[size_t(k) * n_total + n] = st[k];
+		for(int k = 0; k < NS; ++k) nd.state[size_t(k) * n_total + n] = st[k];
 #pragma unroll
 		for(int k = 0; k < 4; ++k) nd.rng[size_t(k) * n_total + n] = rng.s[k];
 		nd.recent[n] = recent;
@@ -627,15 +674,27 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 
 } // namespace
 
+/* The step kernel in four variants: saturating accumulation or not, and
+ * Izhikevich-only networks or any built-in model. Each variant is its own
+ * kernel so that each gets its own register allocation; the host launches
+ * the variant of the network (a batched launch packs networks of one
+ * variant). */
+template<bool SAT, bool IZO>
 __global__ void __launch_bounds__(1024)
 nemo_step_kernel(const NetDev* __restrict__ nets, const WorkItem* __restrict__ work)
 {
 	const WorkItem wi = work[blockIdx.x];
 	const NetDev nd = nets[wi.net];
-	if(nd.flags & NEMO_NET_SATURATE) {
-		run_partition<true>(nd, wi.part);
-	} else {
-		run_partition<false>(nd, wi.part);
+	run_partition<SAT, IZO>(nd, wi.part);
+}
+
+static const void* step_kernel_variant(uint32_t variant)
+{
+	switch(variant & 3u) {
+	case 0: return reinterpret_cast<const void*>(&nemo_step_kernel<false, false>);
+	case 1: return reinterpret_cast<const void*>(&nemo_step_kernel<true, false>);
+	case 2: return reinterpret_cast<const void*>(&nemo_step_kernel<false, true>);
+	default: return reinterpret_cast<const void*>(&nemo_step_kernel<true, true>);
 	}
 }
 
@@ -721,13 +780,13 @@ static unsigned grid_for(size_t n, unsigned block)
 extern "C" {
 
 int nemo_dev_step_launch(const NetDev* nets_dev, const WorkItem* work_dev, uint32_t nblocks,
-		uint32_t blockDim, uint32_t ldsBytes, void* stream)
+		uint32_t blockDim, uint32_t ldsBytes, uint32_t variant, void* stream)
 {
-	hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&nemo_step_kernel),
-			hipFuncAttributeMaxDynamicSharedMemorySize, int(ldsBytes));
+	const void* kern = step_kernel_variant(variant);
+	hipError_t e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, int(ldsBytes));
 	if(e != hipSuccess) return int(e);
 	void* args[] = { const_cast<NetDev**>(&nets_dev), const_cast<WorkItem**>(&work_dev) };
-	e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&nemo_step_kernel),
+	e = hipLaunchCooperativeKernel(kern,
 			dim3(nblocks), dim3(blockDim), args, ldsBytes, static_cast<hipStream_t>(stream));
 	return int(e);
 }
@@ -737,11 +796,12 @@ int nemo_dev_step_launch(const NetDev* nets_dev, const WorkItem* work_dev, uint3
  * work_dev, so one instantiated graph serves every launch of the same
  * work table; the host rewrites nets_dev before each replay. */
 int nemo_dev_batch_graph_create(const NetDev* nets_dev, uint32_t nnets, const WorkItem* work_dev,
-		uint32_t nblocks, uint32_t blockDim, uint32_t ldsBytes, uint32_t* collect_dev, void** exec_out)
+		uint32_t nblocks, uint32_t blockDim, uint32_t ldsBytes, uint32_t variant, uint32_t* collect_dev,
+		void** exec_out)
 {
 	*exec_out = nullptr;
-	hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&nemo_step_kernel),
-			hipFuncAttributeMaxDynamicSharedMemorySize, int(ldsBytes));
+	const void* kern = step_kernel_variant(variant);
+	hipError_t e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, int(ldsBytes));
 	if(e != hipSuccess) return int(e);
 	hipGraph_t g = nullptr;
 	e = hipGraphCreate(&g, 0);
@@ -761,7 +821,7 @@ int nemo_dev_batch_graph_create(const NetDev* nets_dev, uint32_t nnets, const Wo
 
 	void* stepArgs[] = { const_cast<NetDev**>(&nets_dev), const_cast<WorkItem**>(&work_dev) };
 	if(e == hipSuccess) {
-		kp.func = reinterpret_cast<void*>(&nemo_step_kernel);
+		kp.func = const_cast<void*>(kern);
 		kp.gridDim = dim3(nblocks);
 		kp.blockDim = dim3(blockDim);
 		kp.sharedMemBytes = ldsBytes;
@@ -802,14 +862,13 @@ int nemo_dev_graph_destroy(void* exec)
 	return exec ? int(hipGraphExecDestroy(static_cast<hipGraphExec_t>(exec))) : 0;
 }
 
-int nemo_dev_step_max_blocks(uint32_t blockDim, uint32_t ldsBytes, uint32_t* maxBlocks)
+int nemo_dev_step_max_blocks(uint32_t blockDim, uint32_t ldsBytes, uint32_t variant, uint32_t* maxBlocks)
 {
-	hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&nemo_step_kernel),
-			hipFuncAttributeMaxDynamicSharedMemorySize, int(ldsBytes));
+	const void* kern = step_kernel_variant(variant);
+	hipError_t e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, int(ldsBytes));
 	if(e != hipSuccess) return int(e);
 	int perCU = 0;
-	e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU,
-			reinterpret_cast<const void*>(&nemo_step_kernel), int(blockDim), size_t(ldsBytes));
+	e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kern, int(blockDim), size_t(ldsBytes));
 	if(e != hipSuccess) return int(e);
 	int dev = 0;
 	e = hipGetDevice(&dev);

@@ -33,6 +33,7 @@ namespace {
 
 struct BatchPlan {
 	int device = 0;
+	uint32_t variant = 0;
 	uint32_t blockDim = 0;
 	uint32_t lds = 0;
 	uint32_t nnets = 0;
@@ -42,9 +43,10 @@ struct BatchPlan {
 	uint32_t* hcollect = nullptr;  /* pinned, [nnets][4] */
 	void* exec = nullptr;
 
-	bool matches(int dev, uint32_t bd, uint32_t ld, uint32_t nn, const std::vector<WorkItem>& w) const
+	bool matches(int dev, uint32_t var, uint32_t bd, uint32_t ld, uint32_t nn, const std::vector<WorkItem>& w) const
 	{
-		if(dev != device || bd != blockDim || ld != lds || nn != nnets || w.size() != work.size()) return false;
+		if(dev != device || var != variant || bd != blockDim || ld != lds || nn != nnets
+				|| w.size() != work.size()) return false;
 		for(size_t i = 0; i < w.size(); ++i) {
 			if(w[i].net != work[i].net || w[i].part != work[i].part) return false;
 		}
@@ -77,16 +79,18 @@ batchStream(int dev)
 }
 
 BatchPlan&
-getPlan(int dev, uint32_t blockDim, uint32_t lds, uint32_t nnets, const std::vector<WorkItem>& work)
+getPlan(int dev, uint32_t variant, uint32_t blockDim, uint32_t lds, uint32_t nnets,
+		const std::vector<WorkItem>& work)
 {
 	for(auto it = g_plans->begin(); it != g_plans->end(); ++it) {
-		if((*it)->matches(dev, blockDim, lds, nnets, work)) {
+		if((*it)->matches(dev, variant, blockDim, lds, nnets, work)) {
 			g_plans->splice(g_plans->begin(), *g_plans, it);
 			return *g_plans->front();
 		}
 	}
 	std::unique_ptr<BatchPlan> plan(new BatchPlan());
 	plan->device = dev;
+	plan->variant = variant;
 	plan->blockDim = blockDim;
 	plan->lds = lds;
 	plan->nnets = nnets;
@@ -97,7 +101,7 @@ getPlan(int dev, uint32_t blockDim, uint32_t lds, uint32_t nnets, const std::vec
 	hipCheck(hipHostMalloc(reinterpret_cast<void**>(&plan->hnets), size_t(nnets) * sizeof(NetDev)), "hipHostMalloc");
 	hipCheck(hipHostMalloc(reinterpret_cast<void**>(&plan->hcollect), size_t(nnets) * 16), "hipHostMalloc");
 	hipCheck(nemo_dev_batch_graph_create(plan->dnets.as<NetDev>(), nnets, plan->dwork.as<WorkItem>(),
-				uint32_t(work.size()), blockDim, lds, plan->dcollect.as<uint32_t>(), &plan->exec),
+				uint32_t(work.size()), blockDim, lds, variant, plan->dcollect.as<uint32_t>(), &plan->exec),
 			"batched step graph");
 	g_plans->push_front(std::move(plan));
 	while(g_plans->size() > MAX_PLANS) {
@@ -112,40 +116,41 @@ void
 runBatch(Simulation* const sims[], size_t n, const unsigned nsteps[], unsigned long long events[])
 {
 	std::lock_guard<std::mutex> lock(g_planMutex);
-	size_t i = 0;
-	while(i < n) {
-		Simulation* first = sims[i];
-		const unsigned ns0 = nsteps ? nsteps[i] : 0;
-		if(!first->fused() || first->partitions() == 0 || ns0 == 0) {
-			/* plugin-path and empty work run on their own, in order */
+	/* Plugin-path simulations and empty work run on their own; the rest are
+	 * grouped by (device, kernel variant), keeping the given order within a
+	 * group, and packed greedily into launches. */
+	std::vector<size_t> order;
+	for(size_t i = 0; i < n; ++i) {
+		Simulation* sim = sims[i];
+		const unsigned ns = nsteps ? nsteps[i] : 0;
+		if(!sim->fused() || sim->partitions() == 0 || ns == 0) {
 			unsigned long long ev = 0;
-			first->run(ns0, nullptr, &ev);
+			sim->run(ns, nullptr, &ev);
 			if(events) events[i] = ev;
-			++i;
-			continue;
+		} else {
+			order.push_back(i);
 		}
+	}
+	std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) {
+		const int dx = sims[x]->device(), dy = sims[y]->device();
+		return dx != dy ? dx < dy : sims[x]->variant() < sims[y]->variant();
+	});
+	size_t i = 0;
+	while(i < order.size()) {
+		Simulation* first = sims[order[i]];
 		const int dev = first->device();
+		const uint32_t variant = first->variant();
 		hipCheck(hipSetDevice(dev), "hipSetDevice");
-		/* greedy pack of fused networks on the same device */
 		uint32_t blockDim = 64, lds = 0, blocks = 0;
 		size_t j = i;
 		std::vector<size_t> members;
-		while(j < n) {
-			Simulation* sim = sims[j];
-			const unsigned ns = nsteps ? nsteps[j] : 0;
-			if(!sim->fused() || sim->device() != dev) break;
-			if(sim->partitions() == 0 || ns == 0) {
-				/* nothing to launch; advance it in order */
-				unsigned long long ev = 0;
-				sim->run(ns, nullptr, &ev);
-				if(events) events[j] = ev;
-				++j;
-				continue;
-			}
+		while(j < order.size()) {
+			Simulation* sim = sims[order[j]];
+			if(sim->device() != dev || sim->variant() != variant) break;
 			const uint32_t bd = std::max(blockDim, sim->blockThreads());
 			const uint32_t ld = std::max(lds, sim->ldsBytes());
 			uint32_t maxB = 0;
-			hipCheck(nemo_dev_step_max_blocks(bd, ld, &maxB), "occupancy query");
+			hipCheck(nemo_dev_step_max_blocks(bd, ld, variant, &maxB), "occupancy query");
 			if(blocks + sim->partitions() > maxB) {
 				if(members.empty()) {
 					throw exception(NEMO_API_UNSUPPORTED, "simulation does not fit the device");
@@ -155,7 +160,7 @@ runBatch(Simulation* const sims[], size_t n, const unsigned nsteps[], unsigned l
 			blockDim = bd;
 			lds = ld;
 			blocks += sim->partitions();
-			members.push_back(j);
+			members.push_back(order[j]);
 			++j;
 		}
 		hipStream_t st = batchStream(dev);
@@ -168,7 +173,7 @@ runBatch(Simulation* const sims[], size_t n, const unsigned nsteps[], unsigned l
 			for(uint32_t p = 0; p < sim->partitions(); ++p) items.push_back(std::make_pair(idx, p));
 		}
 		const std::vector<WorkItem> work = Simulation::xcdWorkTable(items);
-		BatchPlan& plan = getPlan(dev, blockDim, lds, uint32_t(nets.size()), work);
+		BatchPlan& plan = getPlan(dev, variant, blockDim, lds, uint32_t(nets.size()), work);
 		std::memcpy(plan.hnets, nets.data(), nets.size() * sizeof(NetDev));
 		hipCheck(hipMemcpyAsync(plan.dnets.ptr, plan.hnets, nets.size() * sizeof(NetDev),
 					hipMemcpyHostToDevice, st), "H2D");

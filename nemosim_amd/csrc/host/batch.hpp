@@ -13,9 +13,9 @@ namespace hip {
 
 class Simulation;
 
-/* Runs nsteps[i] cycles of sims[i] for every i. Networks are packed greedily
- * in the given order while their workgroups fit co-resident; simulations on
- * the plugin path run on their own. events[i] (if events) receives the
+/* Runs nsteps[i] cycles of sims[i] for every i. Networks of one device and
+ * step-kernel variant are packed greedily, in the given order, while their
+ * workgroups fit co-resident; simulations on the plugin path run on their own. events[i] (if events) receives the
  * delivered-synapse count of sims[i]. */
 void runBatch(Simulation* const sims[], size_t n, const unsigned nsteps[], unsigned long long events[]);
 

@@ -267,9 +267,11 @@ Simulation::Simulation(const NetworkImpl& net, const ConfigurationImpl& conf) :
 	m_T = T;
 	m_P = (m_n + T - 1) / T;
 	m_lds = ldsFor(T);
+	m_izOnly = !m_model.empty() && std::all_of(m_model.begin(), m_model.end(),
+			[](uint8_t m) { return m == uint8_t(nemo_dev::MODEL_IZHIKEVICH); });
 	if(whyNotFused.empty() && m_P > 0) {
 		uint32_t maxBlocks = 0;
-		hipCheck(nemo_dev_step_max_blocks(m_T, m_lds, &maxBlocks), "occupancy query");
+		hipCheck(nemo_dev_step_max_blocks(m_T, m_lds, variant(), &maxBlocks), "occupancy query");
 		if(m_P > maxBlocks) {
 			whyNotFused = fmt("network of ", m_n, " neurons needs ", m_P,
 					" co-resident workgroups of ", m_T, " neurons; the device holds ", maxBlocks);
@@ -636,7 +638,8 @@ Simulation::prepareLaunch(unsigned nsteps, void* stream, bool clearControl)
 	nd.D = m_D;
 	nd.words = m_words;
 	nd.flags = (m_stdp ? NEMO_NET_STDP : 0u) | (m_saturate ? NEMO_NET_SATURATE : 0u)
-		| (m_replay ? NEMO_NET_REPLAY : 0u) | (m_defer ? NEMO_NET_DEFER : 0u);
+		| (m_replay ? NEMO_NET_REPLAY : 0u) | (m_defer ? NEMO_NET_DEFER : 0u)
+		| (m_izOnly ? NEMO_NET_IZHIKEVICH_ONLY : 0u);
 	nd.nsteps = nsteps;
 	nd.spinLimit = m_spinLimit;
 	nd.glog2 = m_glog2;
@@ -808,7 +811,7 @@ Simulation::launch(unsigned nsteps, unsigned long long* events)
 	hipCheck(hipEventCreate(&e0), "hipEventCreate");
 	hipCheck(hipEventCreate(&e1), "hipEventCreate");
 	hipCheck(hipEventRecord(e0, st), "hipEventRecord");
-	hipCheck(nemo_dev_step_launch(d_net.as<NetDev>(), d_work.as<WorkItem>(), m_P, m_T, m_lds, st),
+	hipCheck(nemo_dev_step_launch(d_net.as<NetDev>(), d_work.as<WorkItem>(), m_P, m_T, m_lds, variant(), st),
 			"step kernel launch");
 	hipCheck(hipEventRecord(e1, st), "hipEventRecord");
 	finishLaunch(nsteps, m_stream, events);

@@ -106,6 +106,8 @@ class Simulation
 		void completeLaunch(unsigned nsteps, uint32_t status);
 		unsigned neuronCount() const { return m_n; }
 		int device() const { return m_device; }
+		/* step-kernel variant (nemo_step_variant): saturating, Izhikevich-only */
+		uint32_t variant() const { return (m_saturate ? 1u : 0u) | (m_izOnly ? 2u : 0u); }
 		/* true: the fused persistent kernel; false: plugin kernels per cycle */
 		bool fused() const { return m_fused; }
 
@@ -141,6 +143,8 @@ class Simulation
 		uint32_t m_glog2 = 0;
 		/* fused step delivers delay >= 2 records one cycle late (NEMO_NET_DEFER) */
 		bool m_defer = false;
+		/* every neuron is an Izhikevich neuron (NEMO_NET_IZHIKEVICH_ONLY) */
+		bool m_izOnly = false;
 
 		/* index maps */
 		std::vector<unsigned> m_localToGlobal;
