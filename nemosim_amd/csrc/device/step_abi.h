@@ -24,7 +24,14 @@ extern "C" {
                                  * delay >= 2 are delivered one cycle later */
 
 /* Bits of the per-network status word */
-#define NEMO_STATUS_BARRIER_TIMEOUT 1u
+#define NEMO_STATUS_BARRIER_TIMEOUT 1u   /* a fired count never arrived */
+#define NEMO_STATUS_ENTRY_TIMEOUT   2u   /* a fired-list entry never arrived */
+
+/* The per-network control block (16 words): [0] workgroups started, [1]
+ * status, [2..3] delivered-synapse count, and, for the first timeout,
+ * [4] cycle, [5] waiting partition, [6] awaited partition, [7] workgroups
+ * started by then. */
+#define NEMO_CTL_WORDS 16u
 
 /*
  * One simulated network as seen by the step kernel. Neurons use
@@ -102,8 +109,8 @@ int nemo_dev_step_launch(const NetDev* nets_dev, const WorkItem* work_dev, uint3
 		uint32_t blockDim, uint32_t ldsBytes, uint32_t variant, void* stream);
 int nemo_dev_step_max_blocks(uint32_t blockDim, uint32_t ldsBytes, uint32_t variant, uint32_t* maxBlocks);
 /* Batched step as an instantiated HIP graph (clear -> cooperative step ->
- * collect); collect_dev receives [nnets][4] words {status, 0, events lo, hi}.
- * The graph reads nets_dev/work_dev at replay time. */
+ * collect); collect_dev receives [nnets][8] words: the first 8 words of each
+ * network's control block. The graph reads nets_dev/work_dev at replay time. */
 int nemo_dev_batch_graph_create(const NetDev* nets_dev, uint32_t nnets, const WorkItem* work_dev,
 		uint32_t nblocks, uint32_t blockDim, uint32_t ldsBytes, uint32_t variant, uint32_t* collect_dev,
 		void** exec_out);
@@ -120,7 +127,10 @@ int nemo_dev_nrand_seq(const uint32_t* state4_dev, float* out_dev, size_t n, voi
 
 /* LDS bytes the step kernel needs for partitions of T neurons, ring depth D,
  * P partitions. */
-static inline uint32_t nemo_step_lds_bytes(uint32_t T, uint32_t D, uint32_t P, int saturate)
+/* Bytes of LDS the STDP tables take (the StdpParams of models.h, rounded) */
+#define NEMO_STDP_LDS_BYTES 576u
+
+static inline uint32_t nemo_step_lds_bytes(uint32_t T, uint32_t D, uint32_t P, int saturate, int stdp)
 {
 	uint32_t b = 0;
 	b += 2u * D * T * 4u;          /* ringE, ringI */
@@ -130,6 +140,10 @@ static inline uint32_t nemo_step_lds_bytes(uint32_t T, uint32_t D, uint32_t P, i
 	b += 64u * 4u;                 /* wave sums / scratch */
 	if(saturate) {
 		b += 2u * D * (T / 32u) * 4u;  /* overflow bits */
+	}
+	if(stdp) {
+		b += NEMO_STDP_LDS_BYTES;      /* STDP tables */
+		b += (2u * T + 64u) * 4u;      /* row starts and prefix of the STDP targets */
 	}
 	return b;
 }

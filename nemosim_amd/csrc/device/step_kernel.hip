@@ -46,6 +46,8 @@
 
 using namespace nemo_dev;
 
+static_assert(sizeof(StdpParams) <= NEMO_STDP_LDS_BYTES, "STDP tables must fit their LDS area");
+
 namespace {
 
 __device__ inline uint32_t lds_prefix_waves(uint32_t* wsum, uint32_t wave, uint32_t nwaves, uint32_t* total)
@@ -68,8 +70,10 @@ __device__ inline uint32_t lds_prefix_waves(uint32_t* wsum, uint32_t wave, uint3
  * publishing the count that names them. */
 #if defined(__HIP_DEVICE_COMPILE__)
 #define NEMO_GLOBAL __attribute__((address_space(1)))
+#define NEMO_LDS __attribute__((address_space(3)))
 #else
 #define NEMO_GLOBAL   /* host pass: the kernels are only parsed there */
+#define NEMO_LDS
 #endif
 
 /* The network's arrays as global-address-space pointers. Pointers read from
@@ -105,8 +109,22 @@ __device__ inline V ld_agent(const NEMO_GLOBAL V* p)
  * wait of the launch fall through, so all waves run to completion. */
 constexpr uint32_t GATHER_CH = 4u;
 
+/* Flags a timed-out wait in the status word; the first one also records
+ * where it happened (control block words 4..7, see step_abi.h). */
+__device__ inline void report_timeout(uint32_t* status, uint32_t kind, uint32_t cycle, uint32_t part,
+		uint32_t awaited)
+{
+	const uint32_t old = atomicOr(status, kind);
+	if(old == 0u) {
+		status[3] = cycle;
+		status[4] = part;
+		status[5] = awaited;
+		status[6] = ld_agent(glb(status - 1));
+	}
+}
+
 __device__ inline void gather_counts(const NEMO_GLOBAL uint64_t* fc, uint32_t P, uint32_t tag, uint32_t* pref,
-		uint32_t lane, uint32_t spinLimit, uint32_t* status)
+		uint32_t lane, uint32_t spinLimit, uint32_t* status, uint32_t part)
 {
 	uint32_t carry = 0;
 	for(uint32_t c0 = 0; c0 < P; c0 += 64u * GATHER_CH) {
@@ -132,12 +150,17 @@ __device__ inline void gather_counts(const NEMO_GLOBAL uint64_t* fc, uint32_t P,
 				}
 			}
 			if(!pending) break;
-			if(ld_agent(glb(status)) != 0u) break;
-			__builtin_amdgcn_s_sleep(1);
-			if(++spins > spinLimit) {
-				atomicOr(status, NEMO_STATUS_BARRIER_TIMEOUT);
-				break;
+			/* the status word is read every 16th poll only: each read is
+			 * another round trip in the wait */
+			if((++spins & 15u) == 0u) {
+				if(ld_agent(glb(status)) != 0u) break;
+				if(spins > spinLimit) {
+					report_timeout(status, NEMO_STATUS_BARRIER_TIMEOUT, tag, part,
+							c0 + 64u * uint32_t(__builtin_ctz(pending)) + lane);
+					break;
+				}
 			}
+			__builtin_amdgcn_s_sleep(1);
 		}
 #pragma unroll
 		for(uint32_t i = 0; i < GATHER_CH; ++i) {
@@ -155,23 +178,24 @@ __device__ inline void gather_counts(const NEMO_GLOBAL uint64_t* fc, uint32_t P,
 	if(lane == 0) pref[0] = 0u;
 }
 
-/* A fired-list entry of cycle tag: (tag << 32) | neuron. The count that
- * names it may become visible first, so a stale entry (of cycle tag-2) is
- * read again, boundedly, as gather_counts waits. */
-__device__ inline uint32_t fired_entry(const NEMO_GLOBAL uint64_t* e, uint32_t tag, uint32_t spinLimit,
-		uint32_t* status)
+/* A fired-list entry of cycle tag: (tag << 32) | neuron, given its first
+ * read v. The count that names it may become visible first, so a stale entry
+ * (of an earlier cycle) is read again, boundedly, as gather_counts waits. */
+__device__ inline uint32_t fired_entry(const NEMO_GLOBAL uint64_t* e, uint64_t v, uint32_t tag,
+		uint32_t spinLimit, uint32_t* status, uint32_t part, uint32_t src_part)
 {
-	uint64_t v = ld_agent(e);
 	uint32_t spins = 0;
 	while(uint32_t(v >> 32) != tag) {
-		if(ld_agent(glb(status)) != 0u) {
-			return 0u;
+		if((++spins & 15u) == 0u) {
+			if(ld_agent(glb(status)) != 0u) {
+				return 0u;
+			}
+			if(spins > spinLimit) {
+				report_timeout(status, NEMO_STATUS_ENTRY_TIMEOUT, tag, part, src_part);
+				return 0u;
+			}
 		}
 		__builtin_amdgcn_s_sleep(1);
-		if(++spins > spinLimit) {
-			atomicOr(status, NEMO_STATUS_BARRIER_TIMEOUT);
-			return 0u;
-		}
 		v = ld_agent(e);
 	}
 	return uint32_t(v);
@@ -192,9 +216,10 @@ __device__ inline uint32_t ld_off(const NEMO_GLOBAL uint32_t* o)
  * 4 bits, the number of delay-1 records at the head of the bucket. */
 constexpr uint32_t OFF_NEAR_SHIFT = 28u;
 
-/* Delivery rounds whose record loads are issued together, and records per
- * lane per round (lane i of a group reads records i and i+G) */
-constexpr uint32_t DELIVER_RU = 8u;
+/* Delivery rounds whose record loads are issued together (fewer with the
+ * saturating accumulation, whose returning atomics need more registers),
+ * and records per lane per round (lane i of a group reads records i and i+G) */
+template<bool SAT> struct DeliverRounds { static constexpr uint32_t value = SAT ? 4u : 8u; };
 constexpr uint32_t DELIVER_KPL = 2u;
 
 /* Partition whose fired list holds flat index j: the q with
@@ -230,27 +255,31 @@ __device__ inline void ring_add(int32_t* ringE, int32_t* ringI, uint32_t* ovf, u
 	}
 }
 
-/* Delivers the records [a, b) of up to 128 sources held one per lane
- * (a0/b0 for sources 0..63, a1/b1 for 64..127 of the batch) into the ring,
- * relative to base slot `base`. Groups of G = 2^glog lanes share one source:
- * lane i of a group reads records i and i+G of the bucket, so a load
- * instruction touches a few cache lines instead of 64, and DELIVER_RU rounds
- * of record loads are issued before their LDS atomics. Records from 2G on
- * (rare when G is about 1.5x the mean bucket) are read one per lane per
- * step. All lanes of the wave must call it together (nsrc is wave-uniform). */
+/* One pass of record delivery for up to 128 sources held one per lane
+ * (a0/b0 for sources 0..63, a1/b1 for 64..127 of the batch): groups of
+ * G = 2^glog lanes share one source, lane i of a group reads records i and
+ * i+G of its bucket, so a load instruction touches a few cache lines
+ * instead of 64; DELIVER_RU rounds of record loads are issued before their
+ * LDS atomics. Rounds [rbeg, rend) of the pass_rounds(nsrc) are done here,
+ * so a pass can be split; records from 2G on are left to the caller. */
+__device__ inline uint32_t pass_rounds(uint32_t nsrc, uint32_t glog)
+{
+	return (nsrc + (64u >> glog) - 1u) >> (6u - glog);
+}
+
 template<bool SAT>
-__device__ inline void deliver_ranges(uint32_t a0, uint32_t b0, uint32_t a1, uint32_t b1, uint32_t nsrc,
+__device__ inline void deliver_pass(uint32_t a0, uint32_t b0, uint32_t a1, uint32_t b1, uint32_t nsrc,
 		uint32_t glog, uint32_t lane, uint32_t base, const NEMO_GLOBAL uint2* syn,
-		int32_t* ringE, int32_t* ringI, uint32_t* ovf, uint32_t D, uint32_t T, uint32_t fwords)
+		int32_t* ringE, int32_t* ringI, uint32_t* ovf, uint32_t D, uint32_t T, uint32_t fwords,
+		uint32_t rbeg, uint32_t rend)
 {
 	const uint32_t G = 1u << glog;
 	const uint32_t NG = 64u >> glog;                 /* sources per round */
 	const uint32_t grp = lane >> glog;
 	const uint32_t gl = lane & (G - 1u);
-	const uint32_t rounds = (nsrc + NG - 1u) >> (6u - glog);
-	const uint32_t covered = DELIVER_KPL * G;        /* records per bucket taken by the rounds */
-	const bool anyLong = (__ballot((b0 - a0 > covered) || (b1 - a1 > covered)) != 0ull);
-	for(uint32_t r0 = 0; r0 < rounds; r0 += DELIVER_RU) {    /* wave-uniform */
+	const uint32_t rounds = rend;
+	constexpr uint32_t DELIVER_RU = DeliverRounds<SAT>::value;
+	for(uint32_t r0 = rbeg; r0 < rounds; r0 += DELIVER_RU) {    /* wave-uniform */
 		uint2 rec[DELIVER_RU][DELIVER_KPL];
 		uint32_t ok = 0;
 #pragma unroll
@@ -282,23 +311,57 @@ __device__ inline void deliver_ranges(uint32_t a0, uint32_t b0, uint32_t a1, uin
 				}
 			}
 		}
-		if(anyLong) {
-			/* records 2G, 2G+G, ... of buckets longer than 2G */
-			for(uint32_t u = 0; u < DELIVER_RU && r0 + u < rounds; ++u) {
-				const uint32_t k = (r0 + u) * NG + grp;
-				const bool hi = ((r0 + u) * NG) >= 64u;
-				const uint32_t ak = __shfl(hi ? a1 : a0, int(k & 63u), 64);
-				const uint32_t bk = __shfl(hi ? b1 : b0, int(k & 63u), 64);
-				if(k < nsrc) {
-					for(uint32_t q = ak + covered + gl; q < bk; q += G) {
-						const uint2 r = ld_rec(syn, q);
-						uint32_t sl = base + (r.x >> 16);
-						sl = (sl >= D) ? sl - D : sl;
-						ring_add<SAT>(ringE, ringI, ovf, D, T, fwords, sl, r.x & 0xffffu, int32_t(r.y));
-					}
-				}
-			}
+	}
+}
+
+/* Delivers the records [a, b) of up to 128 sources held one per lane into
+ * the ring, relative to base slot `base`. The first pass takes up to 2G
+ * records of every bucket. The remainders of longer buckets are then packed
+ * into the first lanes (each lane pushes its range to the lane of its rank
+ * among the long ones) and delivered by further passes, so a long bucket
+ * costs one more round of loads for the whole wave, not one per bucket.
+ * All lanes of the wave must call it together (nsrc is wave-uniform). */
+template<bool SAT>
+__device__ inline void deliver_ranges(uint32_t a0, uint32_t b0, uint32_t a1, uint32_t b1, uint32_t nsrc,
+		uint32_t glog, uint32_t lane, uint32_t base, const NEMO_GLOBAL uint2* syn,
+		int32_t* ringE, int32_t* ringI, uint32_t* ovf, uint32_t D, uint32_t T, uint32_t fwords)
+{
+	const uint32_t covered = DELIVER_KPL << glog;   /* records per bucket taken by a pass */
+	if(lane >= nsrc) { a0 = b0 = 0u; }
+	if(64u + lane >= nsrc) { a1 = b1 = 0u; }
+	for(;;) {
+		deliver_pass<SAT>(a0, b0, a1, b1, nsrc, glog, lane, base, syn, ringE, ringI, ovf, D, T, fwords,
+				0u, pass_rounds(nsrc, glog));
+		/* what the pass left */
+		a0 = (b0 - a0 > covered) ? a0 + covered : b0;
+		a1 = (b1 - a1 > covered) ? a1 + covered : b1;
+		const uint64_t m0 = __ballot(a0 < b0);
+		const uint64_t m1 = __ballot(a1 < b1);
+		const uint32_t n0 = uint32_t(__popcll(m0));
+		const uint32_t nl = n0 + uint32_t(__popcll(m1));
+		if(nl == 0u) {
+			return;
 		}
+		if(nl > 63u) {
+			/* too many to pack into one register: another pass as they are */
+			nsrc = 128u;
+			continue;
+		}
+		/* pack: a long source goes to the lane of its rank; the others to
+		 * lane 63, which no rank reaches */
+		const uint32_t below = uint32_t(__popcll(m0 & ((1ull << lane) - 1ull)));
+		const uint32_t below1 = uint32_t(__popcll(m1 & ((1ull << lane) - 1ull)));
+		const int dst0 = int(((a0 < b0) ? below : 63u) * 4u);
+		const int dst1 = int(((a1 < b1) ? n0 + below1 : 63u) * 4u);
+		const uint32_t pa0 = uint32_t(__builtin_amdgcn_ds_permute(dst0, int(a0)));
+		const uint32_t pb0 = uint32_t(__builtin_amdgcn_ds_permute(dst0, int(b0)));
+		const uint32_t pa1 = uint32_t(__builtin_amdgcn_ds_permute(dst1, int(a1)));
+		const uint32_t pb1 = uint32_t(__builtin_amdgcn_ds_permute(dst1, int(b1)));
+		a0 = (lane < n0) ? pa0 : pa1;
+		b0 = (lane < n0) ? pb0 : pb1;
+		if(lane >= nl) { a0 = b0 = 0u; }
+		a1 = b1 = 0u;
+		nsrc = nl;
 	}
 }
 
@@ -332,10 +395,21 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 	uint32_t* wsum = pref + ((P + 1u + 63u) & ~63u);
 	uint32_t* ovf = wsum + 64;                                 /* [2][D][T/32] when SAT */
 	const uint32_t fwords = T >> 5;
+	/* with STDP: the STDP tables, then the row start and the prefix of the
+	 * row lengths of each target processed in the STDP phase */
+	uint8_t* stdpArea = reinterpret_cast<uint8_t*>(ovf + (SAT ? 2u * D * fwords : 0u));
+	StdpParams* spl = reinterpret_cast<StdpParams*>(stdpArea);
+	uint32_t* qs = reinterpret_cast<uint32_t*>(stdpArea + NEMO_STDP_LDS_BYTES);
+	uint32_t* qp = qs + T;
 	const bool defer = (nd.flags & NEMO_NET_DEFER) != 0u;
 	const uint32_t offMask = defer ? ((1u << OFF_NEAR_SHIFT) - 1u) : ~0u;
 	const NEMO_GLOBAL uint2* syn = glb(reinterpret_cast<const uint2*>(nd.syn));
 	const NEMO_GLOBAL uint32_t* off = glb(nd.off);
+
+	/* count the workgroups of the network that have started (diagnostics) */
+	if(tid == 0) {
+		atomicAdd(nd.status - 1, 1u);
+	}
 
 	/* restore the pending-current ring */
 	const int32_t* gring = nd.ring + size_t(p) * 2u * D * T;
@@ -400,6 +474,12 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 	const bool stdp = (nd.flags & NEMO_NET_STDP) != 0u;
 	const StdpParams* sp = reinterpret_cast<const StdpParams*>(nd.stdp);
 	const uint32_t postLen = stdp ? uint32_t(sp->postLen) : 0u;
+	if(stdp) {
+		/* the tables to LDS: they are read for every plastic synapse */
+		const uint32_t* src = reinterpret_cast<const uint32_t*>(sp);
+		uint32_t* dst = reinterpret_cast<uint32_t*>(spl);
+		for(uint32_t i = tid; i < uint32_t(sizeof(StdpParams) / 4u); i += nthreads) dst[i] = src[i];
+	}
 	unsigned long long events = 0;
 	/* deferred delivery: this wave's far ranges [a, b) (delays >= 2) of the
 	 * previous cycle's spikes, one source per lane, kn sources (wave-uniform) */
@@ -494,6 +574,14 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 				const uint64_t below = lane ? (mask & (~0ull >> (64u - lane))) : 0ull;
 				const uint32_t pos = base + uint32_t(__popcll(below));
 				st_agent(glb(nd.fl) + (size_t(flb) * P + p) * T + pos, (uint64_t(uint32_t(t)) << 32) | uint64_t(n));
+				/* Touch every 128-byte line of this neuron's bucket-offset row
+				 * (loads straight into an LDS sink, no registers held), so
+				 * the other partitions, reading one entry each right after
+				 * the exchange, find the row in the memory-side cache. */
+				const NEMO_GLOBAL uint32_t* row = off + size_t(n) * (P + 1u);
+				for(uint32_t li = 0; li < P + 1u; li += 32u) {
+					__builtin_amdgcn_global_load_lds((NEMO_GLOBAL void*)(row + li), (NEMO_LDS void*)ext, 4, 0, 0);
+				}
 			}
 			/* publish the count. Fired-list entries carry the cycle tag, so
 			 * readers validate each entry and need no ordering against the
@@ -519,7 +607,7 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 
 		/* ---- 3. wait for the network's other partitions */
 		if(wave == 0) {
-			gather_counts(glb(nd.fc) + par * P, P, uint32_t(t), pref, lane, nd.spinLimit, nd.status);
+			gather_counts(glb(nd.fc) + par * P, P, uint32_t(t), pref, lane, nd.spinLimit, nd.status, p);
 		}
 		__syncthreads();
 
@@ -544,9 +632,15 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 					const bool v1 = 64u + lane < nsrc;
 					const uint32_t q0 = v0 ? partition_of(pref, P, j0) : 0u;
 					const uint32_t q1 = v1 ? partition_of(pref, P, j1) : 0u;
+					/* entry offsets within the cycle's fired-list buffer */
+					const uint32_t o0 = q0 * T + (j0 - pref[q0]);
+					const uint32_t o1 = q1 * T + (j1 - pref[q1]);
+					uint64_t ev0 = 0, ev1 = 0;
+					if(v0) ev0 = ld_agent(fl + o0);
+					if(v1) ev1 = ld_agent(fl + o1);
 					uint32_t s0 = 0, s1 = 0;
-					if(v0) s0 = fired_entry(fl + size_t(q0) * T + (j0 - pref[q0]), uint32_t(t), nd.spinLimit, nd.status);
-					if(v1) s1 = fired_entry(fl + size_t(q1) * T + (j1 - pref[q1]), uint32_t(t), nd.spinLimit, nd.status);
+					if(v0) s0 = fired_entry(fl + o0, ev0, uint32_t(t), nd.spinLimit, nd.status, p, q0);
+					if(v1) s1 = fired_entry(fl + o1, ev1, uint32_t(t), nd.spinLimit, nd.status, p, q1);
 					if(v0) {
 						const NEMO_GLOBAL uint32_t* o = off + size_t(s0) * (P + 1u) + p;
 						a0 = ld_off(o);
@@ -595,6 +689,10 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 			}
 		}
 
+		/* the row touches above write into `ext`, which the STDP phase and the
+		 * next cycle's stimulus reuse: let them land first */
+		__builtin_amdgcn_s_waitcnt(0);
+
 		/* ---- 5. STDP accumulation for targets that fired postLen cycles ago */
 		if(stdp) {
 			const bool q = active && (((recent >> postLen) & 1ull) != 0ull);
@@ -609,19 +707,71 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 				qlist[base + uint32_t(__popcll(below))] = n;
 			}
 			__syncthreads();
+			/* row start and exclusive prefix of the row lengths of the
+			 * targets, so that all their plastic in-synapses form one flat
+			 * range the whole workgroup strides over */
+			const NEMO_GLOBAL uint32_t* rstart = glb(nd.rcm_start);
+			if(wave == 0) {
+				uint32_t carry = 0;
+				for(uint32_t c = 0; c < nq; c += 64u) {
+					const uint32_t i = c + lane;
+					uint32_t len = 0;
+					if(i < nq) {
+						const uint32_t tg = qlist[i];
+						const uint32_t a = rstart[tg];
+						len = rstart[tg + 1] - a;
+						qs[i] = a;
+					}
+					uint32_t x = len;
+#pragma unroll
+					for(uint32_t d = 1; d < 64u; d <<= 1) {
+						const uint32_t y = __shfl_up(x, d, 64);
+						if(lane >= d) x += y;
+					}
+					if(i < nq) qp[i + 1] = carry + x;
+					carry += __shfl(x, 63, 64);
+				}
+				if(lane == 0) qp[0] = 0u;
+			}
+			__syncthreads();
+			const uint32_t total = qp[nq];
 			const NEMO_GLOBAL uint64_t* rf = glb(nd.rfg) + size_t(par) * n_total;
-			for(uint32_t qi = 0; qi < nq; ++qi) {
-				const uint32_t tgt = qlist[qi];
-				const uint32_t r0 = glb(nd.rcm_start)[tgt];
-				const uint32_t r1 = glb(nd.rcm_start)[tgt + 1];
-				for(uint32_t r = r0 + tid; r < r1; r += nthreads) {
-					const uint32_t sd = glb(nd.rcm_sd)[r];
-					const uint32_t src = sd & 0x3ffffffu;
-					const uint32_t d = (sd >> 26) + 1u;
-					const uint64_t pre = ld_agent(&rf[src]) >> d;
-					const int32_t wd = stdp_weight_change(*sp, pre);
+			const NEMO_GLOBAL uint32_t* rsd = glb(nd.rcm_sd);
+			NEMO_GLOBAL int32_t* racc = glb(nd.rcm_acc);
+			/* STDP_K synapses per thread per round: their reverse records
+			 * and accumulators, then the sources' histories, are loaded
+			 * together (src/nemo/ConnectivityMatrix.cpp:193-227) */
+			constexpr uint32_t STDP_K = 4u;
+			for(uint32_t e0 = 0; e0 < total; e0 += STDP_K * nthreads) {
+				uint32_t r[STDP_K], sd[STDP_K];
+				int32_t acc[STDP_K];
+				uint64_t pre[STDP_K];
+#pragma unroll
+				for(uint32_t k = 0; k < STDP_K; ++k) {
+					const uint32_t e = e0 + k * nthreads + tid;
+					r[k] = 0u;
+					if(e < total) {
+						const uint32_t i = partition_of(qp, nq, e);
+						r[k] = qs[i] + (e - qp[i]);
+					}
+				}
+#pragma unroll
+				for(uint32_t k = 0; k < STDP_K; ++k) {
+					const bool ok = e0 + k * nthreads + tid < total;
+					sd[k] = ok ? rsd[r[k]] : 0u;
+					acc[k] = ok ? racc[r[k]] : 0;
+				}
+#pragma unroll
+				for(uint32_t k = 0; k < STDP_K; ++k) {
+					const bool ok = e0 + k * nthreads + tid < total;
+					pre[k] = ok ? (ld_agent(rf + (sd[k] & 0x3ffffffu)) >> ((sd[k] >> 26) + 1u)) : 0ull;
+				}
+#pragma unroll
+				for(uint32_t k = 0; k < STDP_K; ++k) {
+					const bool ok = e0 + k * nthreads + tid < total;
+					const int32_t wd = ok ? stdp_weight_change(*spl, pre[k]) : 0;
 					if(wd != 0) {
-						glb(nd.rcm_acc)[r] = int32_t(uint32_t(glb(nd.rcm_acc)[r]) + uint32_t(wd));
+						racc[r[k]] = int32_t(uint32_t(acc[k]) + uint32_t(wd));
 					}
 				}
 			}
@@ -698,27 +848,24 @@ static const void* step_kernel_variant(uint32_t variant)
 	}
 }
 
-/* Batched launch, first node: clear each network's status word and event
- * counter (the step kernel only ORs and adds into them). */
+/* Batched launch, first node: clear the first 8 words of each network's
+ * control block (the step kernel only ORs and adds into them). */
 __global__ void nemo_batch_clear_kernel(const NetDev* __restrict__ nets, uint32_t count)
 {
 	for(uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
-		*nets[i].status = 0u;
-		*nets[i].events = 0ull;
+		uint32_t* ctl = nets[i].status - 1;
+		for(uint32_t w = 0; w < 8u; ++w) ctl[w] = 0u;
 	}
 }
 
-/* Batched launch, last node: gather each network's status and event count
- * into one array, out[4*i] = status, out[4*i+2..3] = events, so the host
- * reads all of them with one copy. */
+/* Batched launch, last node: gather the first 8 words of each network's
+ * control block (status, events, diagnostics) into out[8*i .. 8*i+7], so the
+ * host reads all of them with one copy. */
 __global__ void nemo_batch_collect_kernel(const NetDev* __restrict__ nets, uint32_t count, uint32_t* __restrict__ out)
 {
 	for(uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
-		const unsigned long long e = *nets[i].events;
-		out[4 * i] = *nets[i].status;
-		out[4 * i + 1] = 0u;
-		out[4 * i + 2] = uint32_t(e);
-		out[4 * i + 3] = uint32_t(e >> 32);
+		const uint32_t* ctl = nets[i].status - 1;
+		for(uint32_t w = 0; w < 8u; ++w) out[8 * i + w] = ctl[w];
 	}
 }
 

@@ -40,7 +40,7 @@ struct BatchPlan {
 	std::vector<WorkItem> work;
 	DevBuf dnets, dwork, dcollect;
 	NetDev* hnets = nullptr;       /* pinned */
-	uint32_t* hcollect = nullptr;  /* pinned, [nnets][4] */
+	uint32_t* hcollect = nullptr;  /* pinned, [nnets][8] */
 	void* exec = nullptr;
 
 	bool matches(int dev, uint32_t var, uint32_t bd, uint32_t ld, uint32_t nn, const std::vector<WorkItem>& w) const
@@ -97,9 +97,9 @@ getPlan(int dev, uint32_t variant, uint32_t blockDim, uint32_t lds, uint32_t nne
 	plan->work = work;
 	plan->dnets.alloc(size_t(nnets) * sizeof(NetDev));
 	plan->dwork.upload(work.data(), work.size() * sizeof(WorkItem));
-	plan->dcollect.alloc(size_t(nnets) * 16);
+	plan->dcollect.alloc(size_t(nnets) * 32);
 	hipCheck(hipHostMalloc(reinterpret_cast<void**>(&plan->hnets), size_t(nnets) * sizeof(NetDev)), "hipHostMalloc");
-	hipCheck(hipHostMalloc(reinterpret_cast<void**>(&plan->hcollect), size_t(nnets) * 16), "hipHostMalloc");
+	hipCheck(hipHostMalloc(reinterpret_cast<void**>(&plan->hcollect), size_t(nnets) * 32), "hipHostMalloc");
 	hipCheck(nemo_dev_batch_graph_create(plan->dnets.as<NetDev>(), nnets, plan->dwork.as<WorkItem>(),
 				uint32_t(work.size()), blockDim, lds, variant, plan->dcollect.as<uint32_t>(), &plan->exec),
 			"batched step graph");
@@ -116,6 +116,7 @@ void
 runBatch(Simulation* const sims[], size_t n, const unsigned nsteps[], unsigned long long events[])
 {
 	std::lock_guard<std::mutex> lock(g_planMutex);
+	std::lock_guard<std::recursive_mutex> devLock(deviceMutex());
 	/* Plugin-path simulations and empty work run on their own; the rest are
 	 * grouped by (device, kernel variant), keeping the given order within a
 	 * group, and packed greedily into launches. */
@@ -178,7 +179,7 @@ runBatch(Simulation* const sims[], size_t n, const unsigned nsteps[], unsigned l
 		hipCheck(hipMemcpyAsync(plan.dnets.ptr, plan.hnets, nets.size() * sizeof(NetDev),
 					hipMemcpyHostToDevice, st), "H2D");
 		hipCheck(nemo_dev_graph_launch(plan.exec, st), "batched step graph launch");
-		hipCheck(hipMemcpyAsync(plan.hcollect, plan.dcollect.ptr, nets.size() * 16,
+		hipCheck(hipMemcpyAsync(plan.hcollect, plan.dcollect.ptr, nets.size() * 32,
 					hipMemcpyDeviceToHost, st), "D2H");
 		hipCheck(hipStreamSynchronize(st), "hipStreamSynchronize");
 		/* every member ran: advance all of them, then report the first error */
@@ -186,14 +187,18 @@ runBatch(Simulation* const sims[], size_t n, const unsigned nsteps[], unsigned l
 		exception firstError(NEMO_OK, "");
 		for(size_t m = 0; m < members.size(); ++m) {
 			const size_t k = members[m];
-			const uint32_t* c = plan.hcollect + 4 * m;
+			const uint32_t* c = plan.hcollect + 8 * m;
 			if(events) {
 				events[k] = (unsigned long long) c[2] | ((unsigned long long) c[3] << 32);
 			}
 			try {
-				sims[k]->completeLaunch(nsteps[k], c[0]);
+				sims[k]->completeLaunch(nsteps[k], c);
 			} catch(exception& e) {
-				if(!failed) firstError = e;
+				if(!failed) {
+					firstError = exception(e.errorNumber(), fmt(e.what(), " [batched launch of ", members.size(),
+								" networks, ", work.size(), " workgroups of ", blockDim, " threads, ", lds,
+								" B LDS, variant ", variant, "]"));
+				}
 				failed = true;
 			}
 		}

@@ -176,4 +176,12 @@ hipCheck(int err, const char* what)
 	}
 }
 
+std::recursive_mutex&
+deviceMutex()
+{
+	/* never destroyed: threads may still hold it while the process exits */
+	static std::recursive_mutex* m = new std::recursive_mutex();
+	return *m;
+}
+
 } // namespace nemo

@@ -1,6 +1,7 @@
 #ifndef NEMOSIM_HOST_COMMON_HPP
 #define NEMOSIM_HOST_COMMON_HPP
 
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <sstream>
@@ -65,6 +66,14 @@ int32_t fx_toFix(float f);
 float fx_toFloat(int32_t v);
 
 void hipCheck(int err, const char* what);
+
+/* Held around this library's device work -- uploads at construction,
+ * launches, copies -- so that work queued from one host thread (a serving
+ * loop building simulations on worker threads) never shares the device
+ * with a persistent step launch: its workgroups must all be resident at
+ * once, and a copy kernel dispatched into the middle of that launch could
+ * delay some of them. Recursive: a batched run launches single runs. */
+std::recursive_mutex& deviceMutex();
 
 } // namespace nemo
 

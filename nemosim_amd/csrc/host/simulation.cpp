@@ -256,7 +256,7 @@ Simulation::Simulation(const NetworkImpl& net, const ConfigurationImpl& conf) :
 	}
 	auto ldsFor = [&](uint32_t t) {
 		uint32_t p = (m_n + t - 1) / t;
-		return nemo_step_lds_bytes(t, m_D, std::max(1u, p), m_saturate ? 1 : 0);
+		return nemo_step_lds_bytes(t, m_D, std::max(1u, p), m_saturate ? 1 : 0, m_stdp ? 1 : 0);
 	};
 	while(T > 64 && ldsFor(T) > ldsLimit) {
 		T -= 64;
@@ -292,7 +292,9 @@ Simulation::Simulation(const NetworkImpl& net, const ConfigurationImpl& conf) :
 	} else {
 		m_words = m_P * (T / 64u);
 		/* lanes per fired source in delivery: a power of two about 1.5x the
-		 * mean bucket length (synapses of one source into one partition) */
+		 * mean bucket length (synapses of one source into one partition);
+		 * each lane takes two records per pass, so one pass covers almost
+		 * every bucket (measured at 200k x 1000: 8 lanes beat 4) */
 		const double lavg = m_n ? double(m_nsyn) / (double(m_n) * double(m_P)) : 0.0;
 		m_glog2 = 0;
 		if(conf.deliveryLanes) {
@@ -432,6 +434,7 @@ Simulation::Simulation(const NetworkImpl& net, const ConfigurationImpl& conf) :
 	}
 
 	/* ---- upload */
+	std::unique_lock<std::recursive_mutex> devLock(deviceMutex());
 	d_model.upload(m_model.data(), m_model.size());
 	d_param.upload(param.data(), param.size() * sizeof(float));
 	d_state.upload(state.data(), state.size() * sizeof(float));
@@ -484,6 +487,7 @@ Simulation::Simulation(const NetworkImpl& net, const ConfigurationImpl& conf) :
 		loadPlugins(net);
 	}
 	hipCheck(hipStreamSynchronize(s), "hipStreamSynchronize");
+	devLock.unlock();
 
 	m_deviceBytes = d_model.bytes + d_param.bytes + d_state.bytes + d_rng.bytes + d_recent.bytes
 		+ d_ring.bytes + d_off.bytes + d_syn.bytes + d_fl.bytes + d_fc.bytes + d_rfg.bytes
@@ -541,6 +545,7 @@ Simulation::loadPlugins(const NetworkImpl& net)
 
 Simulation::~Simulation()
 {
+	std::lock_guard<std::recursive_mutex> devLock(deviceMutex());
 	if(m_stream) {
 		(void) hipStreamSynchronize(static_cast<hipStream_t>(m_stream));
 		(void) hipStreamDestroy(static_cast<hipStream_t>(m_stream));
@@ -566,6 +571,7 @@ Simulation::localIdx(unsigned g) const
 void
 Simulation::uploadSchedule(unsigned nsteps, const nemo_schedule_t* s)
 {
+	std::lock_guard<std::recursive_mutex> devLock(deviceMutex());
 	m_hasSchedule = false;
 	if(!s) return;
 	std::vector<uint32_t> fsPtr, fsIdx, isPtr, isIdx;
@@ -677,25 +683,28 @@ void
 Simulation::finishLaunch(unsigned nsteps, void* stream, unsigned long long* events)
 {
 	hipStream_t st = static_cast<hipStream_t>(stream);
-	uint32_t ctl[4];
+	uint32_t ctl[8];
 	hipCheck(hipMemcpyAsync(ctl, d_ctl.ptr, sizeof(ctl), hipMemcpyDeviceToHost, st), "D2H");
 	hipCheck(hipStreamSynchronize(st), "hipStreamSynchronize");
 	unsigned long long e;
 	std::memcpy(&e, &ctl[2], sizeof(e));
-	completeLaunch(nsteps, ctl[1]);
 	if(events) *events = e;
+	completeLaunch(nsteps, ctl);
 }
 
 void
-Simulation::completeLaunch(unsigned nsteps, uint32_t status)
+Simulation::completeLaunch(unsigned nsteps, const uint32_t ctl[8])
 {
 	m_cycle += nsteps;
 	m_timerCycles += nsteps;
 	m_lastRunSteps = nsteps;
 	m_hasSchedule = false;
 	m_replay = false;
-	if(status & NEMO_STATUS_BARRIER_TIMEOUT) {
-		throw exception(NEMO_CUDA_ERROR, "step kernel: a workgroup barrier timed out (workgroups not co-resident?)");
+	if(ctl[1] & (NEMO_STATUS_BARRIER_TIMEOUT | NEMO_STATUS_ENTRY_TIMEOUT)) {
+		throw exception(NEMO_CUDA_ERROR, fmt("step kernel: a wait for another workgroup timed out (",
+					(ctl[1] & NEMO_STATUS_BARRIER_TIMEOUT) ? "fired count" : "fired-list entry",
+					"; cycle ", ctl[4], ", partition ", ctl[5], " waiting for partition ", ctl[6],
+					"; ", ctl[7], " of ", m_P, " workgroups had started, ", ctl[0], " by the end)"));
 	}
 }
 
@@ -706,6 +715,7 @@ Simulation::completeLaunch(unsigned nsteps, uint32_t status)
 void
 Simulation::launchPlugins(unsigned nsteps, unsigned long long* events)
 {
+	std::lock_guard<std::recursive_mutex> devLock(deviceMutex());
 	hipStream_t st = static_cast<hipStream_t>(m_stream);
 	hipCheck(hipSetDevice(m_device), "hipSetDevice");
 	d_raster.ensure(std::max<size_t>(size_t(nsteps) * m_words32, 2) * 4);
@@ -769,7 +779,7 @@ Simulation::launchPlugins(unsigned nsteps, unsigned long long* events)
 		}
 	}
 	hipCheck(hipEventRecord(e1, st), "hipEventRecord");
-	uint32_t ctl[4];
+	uint32_t ctl[8];
 	hipCheck(hipMemcpyAsync(ctl, d_ctl.ptr, sizeof(ctl), hipMemcpyDeviceToHost, st), "D2H");
 	hipCheck(hipStreamSynchronize(st), "hipStreamSynchronize");
 	float ms = 0.0f;
@@ -791,6 +801,7 @@ Simulation::launchPlugins(unsigned nsteps, unsigned long long* events)
 void
 Simulation::launch(unsigned nsteps, unsigned long long* events)
 {
+	std::lock_guard<std::recursive_mutex> devLock(deviceMutex());
 	if(events) *events = 0;
 	if(!m_fused && m_n > 0 && nsteps > 0) {
 		launchPlugins(nsteps, events);
@@ -839,6 +850,7 @@ Simulation::decodeStep(const uint64_t* words, std::vector<unsigned>& out) const
 const std::vector<unsigned>&
 Simulation::step(const std::vector<unsigned>& fstim, const unsigned* istimIdx, const float* istimVal, size_t nistim)
 {
+	std::lock_guard<std::recursive_mutex> devLock(deviceMutex());
 	unsigned fp[2] = { 0, unsigned(fstim.size()) };
 	unsigned ip[2] = { 0, unsigned(nistim) };
 	nemo_schedule_t s;
@@ -869,6 +881,7 @@ Simulation::run(unsigned nsteps, const nemo_schedule_t* schedule, unsigned long 
 void
 Simulation::readFired(unsigned counts[], unsigned ids[], size_t cap, size_t* total)
 {
+	std::lock_guard<std::recursive_mutex> devLock(deviceMutex());
 	size_t nw = size_t(m_lastRunSteps) * m_words;
 	m_rasterHost.resize(nw);
 	if(nw) {
@@ -893,6 +906,7 @@ Simulation::readFired(unsigned counts[], unsigned ids[], size_t cap, size_t* tot
 void
 Simulation::readRaster(uint64_t words[], size_t cap, size_t* wordsPerStep)
 {
+	std::lock_guard<std::recursive_mutex> devLock(deviceMutex());
 	size_t nw = size_t(m_lastRunSteps) * m_words;
 	if(wordsPerStep) *wordsPerStep = m_words;
 	if(words) {
@@ -925,6 +939,7 @@ Simulation::info(nemo_hip_info_t* out) const
 void
 Simulation::applyStdp(float reward)
 {
+	std::lock_guard<std::recursive_mutex> devLock(deviceMutex());
 	if(!m_stdp) {
 		throw exception(NEMO_LOGIC_ERROR, "applyStdp called, but no STDP model specified");
 	}
@@ -945,6 +960,7 @@ Simulation::applyStdp(float reward)
 void
 Simulation::readScalar(const DevBuf& b, size_t offset, void* out, size_t bytes) const
 {
+	std::lock_guard<std::recursive_mutex> devLock(deviceMutex());
 	hipCheck(hipSetDevice(m_device), "hipSetDevice");
 	hipCheck(hipMemcpy(out, static_cast<const char*>(b.ptr) + offset, bytes, hipMemcpyDeviceToHost), "D2H");
 }
@@ -952,6 +968,7 @@ Simulation::readScalar(const DevBuf& b, size_t offset, void* out, size_t bytes) 
 void
 Simulation::writeScalar(DevBuf& b, size_t offset, const void* in, size_t bytes)
 {
+	std::lock_guard<std::recursive_mutex> devLock(deviceMutex());
 	hipCheck(hipSetDevice(m_device), "hipSetDevice");
 	hipCheck(hipMemcpy(static_cast<char*>(b.ptr) + offset, in, bytes, hipMemcpyHostToDevice), "H2D");
 }

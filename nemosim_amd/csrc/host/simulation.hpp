@@ -103,7 +103,7 @@ class Simulation
 		/* read back status and events after the launch, then completeLaunch */
 		void finishLaunch(unsigned nsteps, void* stream, unsigned long long* events);
 		/* advance the cycle count by a finished launch; throws on a status error */
-		void completeLaunch(unsigned nsteps, uint32_t status);
+		void completeLaunch(unsigned nsteps, const uint32_t ctl[8]);
 		unsigned neuronCount() const { return m_n; }
 		int device() const { return m_device; }
 		/* step-kernel variant (nemo_step_variant): saturating, Izhikevich-only */
@@ -139,7 +139,10 @@ class Simulation
 		std::chrono::steady_clock::time_point m_timerStart;
 		int m_device = 0;
 		void* m_stream = nullptr;
-		uint32_t m_spinLimit = 1u << 22;
+		/* polls before a wait for another workgroup is declared lost: about
+		 * 0.1-0.2 us each, so seconds -- a safety net against workgroups
+		 * that never run, not a bound any legitimate wait approaches */
+		uint32_t m_spinLimit = 1u << 26;
 		uint32_t m_glog2 = 0;
 		/* fused step delivers delay >= 2 records one cycle late (NEMO_NET_DEFER) */
 		bool m_defer = false;

@@ -239,6 +239,30 @@ def test_deferred_and_immediate_delivery(nemo_mod, oracle_mod, dense):
     _compare_state(sim, orc, spec.n, 2, sample=range(0, spec.n, 7))
 
 
+@pytest.mark.parametrize("lanes", [1, 2, 16, 64])
+def test_delivery_lanes(nemo_mod, oracle_mod, lanes):
+    """Every lanes-per-source setting gives the oracle's raster: with 1 or 2
+    lanes most buckets are longer than one pass covers (their remainders are
+    packed into further passes, more than 63 at a time with 1 lane), with
+    64 lanes most lanes of a group are idle."""
+    spec = examples.izhikevich_80_20(2500, 300, seed=91)
+    net = spec.to_nemo(nemo_mod)
+    conf = nemo_mod.Configuration()
+    conf.set_partition_size(256)
+    conf.set_delivery_lanes(lanes)
+    sim = nemo_mod.Simulation(net, conf)
+    assert sim.info()["lanes_per_source"] == lanes
+    orc = oracle_mod.from_spec(spec)
+    _compare_run(sim, orc, 200)
+    # bursts: 1200 forced sources in one cycle put far more than 63 long
+    # buckets into each wave's batch
+    rng = np.random.default_rng(5)
+    for t in range(40):
+        fs = sorted(rng.choice(2500, size=1200, replace=False).tolist()) if t % 10 == 0 else []
+        assert sim.step(fs) == orc.step(fs), t
+    _compare_run(sim, orc, 100)
+
+
 def test_batch_graph_replays(nemo_mod, oracle_mod):
     """The batched launch is an instantiated HIP graph replayed once per
     call: repeated calls on the same pack (same graph), a changed pack (a
