@@ -741,7 +741,7 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 			/* STDP_K synapses per thread per round: their reverse records
 			 * and accumulators, then the sources' histories, are loaded
 			 * together (src/nemo/ConnectivityMatrix.cpp:193-227) */
-			constexpr uint32_t STDP_K = 4u;
+			constexpr uint32_t STDP_K = IZO ? 8u : 4u;   /* fewer registers left in the generic variant */
 			for(uint32_t e0 = 0; e0 < total; e0 += STDP_K * nthreads) {
 				uint32_t r[STDP_K], sd[STDP_K];
 				int32_t acc[STDP_K];
