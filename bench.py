@@ -195,7 +195,9 @@ def bench_plain(args, stdp=False):
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic_from_profile(args.traffic_file),
+            # measured for the default workload only (profiles/ holds its counters)
+            "traffic": (traffic_from_profile(args.traffic_file)
+                        if (not stdp and n == 200000 and m == 1000 and L == 1000) else None),
             "algorithmic_bytes_per_launch": abytes / launches,
             "kernel_ms_per_launch": kernel_ms / launches,
         },

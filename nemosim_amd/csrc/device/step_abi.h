@@ -48,7 +48,7 @@ typedef struct NetDev {
 	uint32_t nsteps;     /* steps this network runs in the current launch */
 	uint32_t spinLimit;  /* barrier spin bound (iterations) */
 	uint32_t glog2;      /* log2 of the lanes that share one fired source in delivery (0..6) */
-	uint32_t pad0;
+	uint32_t flRing;     /* fired lists and counts kept for this many cycles (buffer t mod flRing) */
 	uint64_t cycle0;     /* cycle count at the start of the launch */
 
 	const uint8_t* model;     /* [n] model id */
@@ -66,9 +66,8 @@ typedef struct NetDev {
 
 	/* per-step exchange between the network's workgroups, double-buffered
 	 * by cycle parity */
-	uint64_t* fl;             /* [3][P][T] fired neurons of each partition, (cycle << 32) | neuron
-	                           * (buffer t mod 2, or t mod 3 with NEMO_NET_DEFER) */
-	uint64_t* fc;             /* [2][P] (cycle << 32) | fired count; starts all-ones */
+	uint64_t* fl;             /* [flRing][P][T] fired neurons of each partition, (cycle << 32) | neuron */
+	uint64_t* fc;             /* [flRing][P] (cycle << 32) | fired count; starts all-ones */
 	uint64_t* rfg;            /* [2][n] firing history (STDP only) */
 	uint32_t* status;         /* NEMO_STATUS_* */
 	unsigned long long* events; /* delivered-synapse counter */
@@ -129,6 +128,8 @@ int nemo_dev_nrand_seq(const uint32_t* state4_dev, float* out_dev, size_t n, voi
  * P partitions. */
 /* Bytes of LDS the STDP tables take (the StdpParams of models.h, rounded) */
 #define NEMO_STDP_LDS_BYTES 576u
+/* Entries of a workgroup's per-cycle STDP work list (64-synapse chunks) */
+#define NEMO_STDP_WORK_CAP 1024u
 
 static inline uint32_t nemo_step_lds_bytes(uint32_t T, uint32_t D, uint32_t P, int saturate, int stdp)
 {
@@ -143,7 +144,7 @@ static inline uint32_t nemo_step_lds_bytes(uint32_t T, uint32_t D, uint32_t P, i
 	}
 	if(stdp) {
 		b += NEMO_STDP_LDS_BYTES;      /* STDP tables */
-		b += (2u * T + 64u) * 4u;      /* row starts and prefix of the STDP targets */
+		b += NEMO_STDP_WORK_CAP * 8u + 64u;  /* STDP work list and its counter */
 	}
 	return b;
 }

@@ -98,6 +98,12 @@ __device__ inline V ld_agent(const NEMO_GLOBAL V* p)
 	return __hip_atomic_load(const_cast<NEMO_GLOBAL V*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+/* device-coherent add (two's-complement wrap, as NeMo's int32 accumulator) */
+__device__ inline void add_agent(NEMO_GLOBAL int32_t* p, int32_t v)
+{
+	(void) __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 /* Wave 0 waits until every partition of the network has published its
  * fired count for cycle t (count tagged with the low 32 bits of t; the
  * buffer of the same parity last held cycle t-2), then writes the exclusive
@@ -389,7 +395,6 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 	int32_t* ringE = reinterpret_cast<int32_t*>(lds);
 	int32_t* ringI = ringE + D * T;
 	float* ext = reinterpret_cast<float*>(ringI + D * T);
-	uint32_t* qlist = reinterpret_cast<uint32_t*>(ext);      /* aliases ext (STDP phase only) */
 	uint32_t* fst = reinterpret_cast<uint32_t*>(ext + T);
 	uint32_t* pref = fst + (T >> 5);
 	uint32_t* wsum = pref + ((P + 1u + 63u) & ~63u);
@@ -399,8 +404,8 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 	 * row lengths of each target processed in the STDP phase */
 	uint8_t* stdpArea = reinterpret_cast<uint8_t*>(ovf + (SAT ? 2u * D * fwords : 0u));
 	StdpParams* spl = reinterpret_cast<StdpParams*>(stdpArea);
-	uint32_t* qs = reinterpret_cast<uint32_t*>(stdpArea + NEMO_STDP_LDS_BYTES);
-	uint32_t* qp = qs + T;
+	uint2* work = reinterpret_cast<uint2*>(stdpArea + NEMO_STDP_LDS_BYTES);   /* [cap] {row index, length} */
+	uint32_t* nwork = reinterpret_cast<uint32_t*>(work + NEMO_STDP_WORK_CAP);
 	const bool defer = (nd.flags & NEMO_NET_DEFER) != 0u;
 	const uint32_t offMask = defer ? ((1u << OFF_NEAR_SHIFT) - 1u) : ~0u;
 	const NEMO_GLOBAL uint2* syn = glb(reinterpret_cast<const uint2*>(nd.syn));
@@ -490,9 +495,11 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 		const uint64_t t = nd.cycle0 + k;
 		const uint32_t slot = uint32_t(t % D);
 		const uint32_t par = uint32_t(t & 1u);
-		/* fired-list buffer: by parity, or of three when deferred delivery
-		 * reads the previous cycle's lists while the next are written */
-		const uint32_t flb = defer ? uint32_t(t % 3u) : par;
+		/* fired-list and count buffer of this cycle: a ring of flRing
+		 * cycles (at least 2; 3 when deferred delivery reads the previous
+		 * cycle's lists while the next are written; postLen+2 with STDP,
+		 * which reads the lists of cycle t-postLen) */
+		const uint32_t flb = uint32_t(t % nd.flRing);
 
 		/* ---- stimulus of this step (sim-local indices, deduplicated by the host) */
 		bool has_stim = false;
@@ -593,7 +600,7 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 				__syncthreads();
 			}
 			if(tid == 0) {
-				st_agent(glb(nd.fc) + par * P + p, (uint64_t(uint32_t(t)) << 32) | uint64_t(total));
+				st_agent(glb(nd.fc) + flb * P + p, (uint64_t(uint32_t(t)) << 32) | uint64_t(total));
 			}
 		}
 
@@ -607,7 +614,7 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 
 		/* ---- 3. wait for the network's other partitions */
 		if(wave == 0) {
-			gather_counts(glb(nd.fc) + par * P, P, uint32_t(t), pref, lane, nd.spinLimit, nd.status, p);
+			gather_counts(glb(nd.fc) + flb * P, P, uint32_t(t), pref, lane, nd.spinLimit, nd.status, p);
 		}
 		__syncthreads();
 
@@ -693,85 +700,90 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 		 * next cycle's stimulus reuse: let them land first */
 		__builtin_amdgcn_s_waitcnt(0);
 
-		/* ---- 5. STDP accumulation for targets that fired postLen cycles ago */
-		if(stdp) {
-			const bool q = active && (((recent >> postLen) & 1ull) != 0ull);
-			const uint64_t qm = __ballot(q);
-			__syncthreads();   /* wsum reuse; also orders delivery before qlist writes */
-			if(lane == 0) wsum[wave] = uint32_t(__popcll(qm));
-			__syncthreads();
-			uint32_t nq;
-			uint32_t base = lds_prefix_waves(wsum, wave, nwaves, &nq);
-			if(q) {
-				const uint64_t below = lane ? (qm & (~0ull >> (64u - lane))) : 0ull;
-				qlist[base + uint32_t(__popcll(below))] = n;
-			}
-			__syncthreads();
-			/* row start and exclusive prefix of the row lengths of the
-			 * targets, so that all their plastic in-synapses form one flat
-			 * range the whole workgroup strides over */
-			const NEMO_GLOBAL uint32_t* rstart = glb(nd.rcm_start);
+		/* ---- 5. STDP accumulation for the targets that fired postLen cycles
+		 * ago (src/nemo/ConnectivityMatrix.cpp:193-227). They are the fired
+		 * list of cycle t-postLen, which every partition still holds (ring
+		 * of postLen+2 cycles); their plastic in-synapse rows are cut into
+		 * chunks of STDP_CHUNK, and chunk c of the j-th target of that list
+		 * is processed by partition (j + c) mod P, so every partition takes
+		 * an equal share whichever partition the targets belong to. */
+		if(stdp && t >= postLen) {
+			constexpr uint32_t STDP_CHUNK = 64u;
+			const uint64_t tc = t - postLen;
+			const uint32_t cb = uint32_t(tc % nd.flRing);
+			__syncthreads();   /* pref reuse; delivery done */
 			if(wave == 0) {
-				uint32_t carry = 0;
-				for(uint32_t c = 0; c < nq; c += 64u) {
-					const uint32_t i = c + lane;
-					uint32_t len = 0;
-					if(i < nq) {
-						const uint32_t tg = qlist[i];
-						const uint32_t a = rstart[tg];
-						len = rstart[tg + 1] - a;
-						qs[i] = a;
+				/* counts of cycle t-postLen, all published long ago */
+				gather_counts(glb(nd.fc) + cb * P, P, uint32_t(tc), pref, lane, nd.spinLimit, nd.status, p);
+			}
+			if(tid == 0) *nwork = 0u;
+			__syncthreads();
+			const uint32_t Fc = pref[P];
+			const NEMO_GLOBAL uint64_t* flc = glb(nd.fl) + size_t(cb) * P * T;
+			const NEMO_GLOBAL uint32_t* rstart = glb(nd.rcm_start);
+			for(uint32_t j = tid; j < Fc; j += nthreads) {
+				const uint32_t q = partition_of(pref, P, j);
+				const NEMO_GLOBAL uint64_t* e = flc + size_t(q) * T + (j - pref[q]);
+				const uint32_t tgt = fired_entry(e, ld_agent(e), uint32_t(tc), nd.spinLimit, nd.status, p, q);
+				const uint32_t r0 = rstart[tgt];
+				const uint32_t len = rstart[tgt + 1] - r0;
+				const uint32_t nch = (len + STDP_CHUNK - 1u) / STDP_CHUNK;
+				for(uint32_t ch = (p + P - j % P) % P; ch < nch; ch += P) {
+					const uint32_t slot = atomicAdd(nwork, 1u);
+					const uint32_t r = r0 + ch * STDP_CHUNK;
+					const uint32_t l = min(STDP_CHUNK, len - ch * STDP_CHUNK);
+					if(slot < NEMO_STDP_WORK_CAP) {
+						work[slot] = make_uint2(r, l);
+					} else {
+						/* list full (bursts on small networks): this thread does it */
+						const NEMO_GLOBAL uint64_t* rf = glb(nd.rfg) + size_t(par) * n_total;
+						for(uint32_t i = 0; i < l; ++i) {
+							const uint32_t sd = glb(nd.rcm_sd)[r + i];
+							const uint64_t pre = ld_agent(rf + (sd & 0x3ffffffu)) >> ((sd >> 26) + 1u);
+							const int32_t wd = stdp_weight_change(*spl, pre);
+							if(wd != 0) {
+								add_agent(glb(nd.rcm_acc) + r + i, wd);
+							}
+						}
 					}
-					uint32_t x = len;
-#pragma unroll
-					for(uint32_t d = 1; d < 64u; d <<= 1) {
-						const uint32_t y = __shfl_up(x, d, 64);
-						if(lane >= d) x += y;
-					}
-					if(i < nq) qp[i + 1] = carry + x;
-					carry += __shfl(x, 63, 64);
 				}
-				if(lane == 0) qp[0] = 0u;
 			}
 			__syncthreads();
-			const uint32_t total = qp[nq];
+			const uint32_t nw = min(*nwork, NEMO_STDP_WORK_CAP);
 			const NEMO_GLOBAL uint64_t* rf = glb(nd.rfg) + size_t(par) * n_total;
 			const NEMO_GLOBAL uint32_t* rsd = glb(nd.rcm_sd);
 			NEMO_GLOBAL int32_t* racc = glb(nd.rcm_acc);
-			/* STDP_K synapses per thread per round: their reverse records
-			 * and accumulators, then the sources' histories, are loaded
-			 * together (src/nemo/ConnectivityMatrix.cpp:193-227) */
+			/* a wave takes STDP_K chunks at once, one synapse per lane of each:
+			 * the reverse records of all of them are loaded together, then the
+			 * sources' histories. The same synapse is accumulated by another
+			 * partition (on another XCD) when its target appears at another
+			 * position of a later cycle's list, so the accumulator is updated
+			 * with a device-scope atomic add, not a cached read and write. */
 			constexpr uint32_t STDP_K = IZO ? 8u : 4u;   /* fewer registers left in the generic variant */
-			for(uint32_t e0 = 0; e0 < total; e0 += STDP_K * nthreads) {
+			for(uint32_t w0 = wave * STDP_K; w0 < nw; w0 += nwaves * STDP_K) {   /* wave-uniform */
 				uint32_t r[STDP_K], sd[STDP_K];
-				int32_t acc[STDP_K];
 				uint64_t pre[STDP_K];
+				bool ok[STDP_K];
 #pragma unroll
 				for(uint32_t k = 0; k < STDP_K; ++k) {
-					const uint32_t e = e0 + k * nthreads + tid;
-					r[k] = 0u;
-					if(e < total) {
-						const uint32_t i = partition_of(qp, nq, e);
-						r[k] = qs[i] + (e - qp[i]);
-					}
+					const uint2 wk = (w0 + k < nw) ? work[w0 + k] : make_uint2(0u, 0u);
+					ok[k] = lane < wk.y;
+					r[k] = wk.x + lane;
 				}
 #pragma unroll
 				for(uint32_t k = 0; k < STDP_K; ++k) {
-					const bool ok = e0 + k * nthreads + tid < total;
-					sd[k] = ok ? rsd[r[k]] : 0u;
-					acc[k] = ok ? racc[r[k]] : 0;
+					/* unconditional loads (entry 0 stands in for absent ones) */
+					sd[k] = rsd[ok[k] ? r[k] : 0u];
 				}
 #pragma unroll
 				for(uint32_t k = 0; k < STDP_K; ++k) {
-					const bool ok = e0 + k * nthreads + tid < total;
-					pre[k] = ok ? (ld_agent(rf + (sd[k] & 0x3ffffffu)) >> ((sd[k] >> 26) + 1u)) : 0ull;
+					pre[k] = ld_agent(rf + (sd[k] & 0x3ffffffu)) >> ((sd[k] >> 26) + 1u);
 				}
 #pragma unroll
 				for(uint32_t k = 0; k < STDP_K; ++k) {
-					const bool ok = e0 + k * nthreads + tid < total;
-					const int32_t wd = ok ? stdp_weight_change(*spl, pre[k]) : 0;
+					const int32_t wd = ok[k] ? stdp_weight_change(*spl, pre[k]) : 0;
 					if(wd != 0) {
-						racc[r[k]] = int32_t(uint32_t(acc[k]) + uint32_t(wd));
+						add_agent(racc + r[k], wd);
 					}
 				}
 			}

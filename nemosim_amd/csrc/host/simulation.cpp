@@ -447,8 +447,12 @@ Simulation::Simulation(const NetworkImpl& net, const ConfigurationImpl& conf) :
 		syn.resize(2, 0u);
 	}
 	d_syn.upload(syn.data(), syn.size() * sizeof(uint32_t));
-	d_fl.alloc(size_t(3) * m_P * m_T * 8);
-	d_fc.alloc(size_t(2) * m_P * 8 + 8);
+	/* fired lists and counts are kept for flRing cycles: 2 (parity), 3 when
+	 * deferred delivery reads the previous cycle's, postLen + 2 with STDP,
+	 * whose accumulation reads those of cycle t - postLen */
+	m_flRing = m_stdp ? std::max(3u, uint32_t(m_sp.postLen) + 2u) : (m_defer ? 3u : 2u);
+	d_fl.alloc(size_t(m_flRing) * m_P * m_T * 8);
+	d_fc.alloc(size_t(m_flRing) * m_P * 8 + 8);
 	d_ctl.alloc(64);
 	if(m_stdp) {
 		d_rfg.alloc(size_t(2) * m_n * 8);
@@ -649,6 +653,7 @@ Simulation::prepareLaunch(unsigned nsteps, void* stream, bool clearControl)
 	nd.nsteps = nsteps;
 	nd.spinLimit = m_spinLimit;
 	nd.glog2 = m_glog2;
+	nd.flRing = m_flRing;
 	nd.cycle0 = m_cycle;
 	nd.model = d_model.as<uint8_t>();
 	nd.param = d_param.as<float>();

@@ -146,6 +146,8 @@ class Simulation
 		uint32_t m_glog2 = 0;
 		/* fused step delivers delay >= 2 records one cycle late (NEMO_NET_DEFER) */
 		bool m_defer = false;
+		/* cycles of fired lists and counts kept on the device */
+		uint32_t m_flRing = 2;
 		/* every neuron is an Izhikevich neuron (NEMO_NET_IZHIKEVICH_ONLY) */
 		bool m_izOnly = false;
 
