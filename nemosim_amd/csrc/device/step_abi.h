@@ -78,6 +78,8 @@ typedef struct NetDev {
 	const uint32_t* rcm_fwd;     /* index of the synapse record */
 	int32_t* rcm_acc;            /* accumulated weight change */
 	const void* stdp;            /* StdpParams */
+	uint64_t* srow;              /* [flRing][P][T] reverse row {start, length} of each fired
+	                              * neuron, at its fired-list position (STDP only) */
 
 	/* stimulus of this launch, CSR over steps (sim-local indices); NULL if none */
 	const uint32_t* fs_ptr;
@@ -144,7 +146,8 @@ static inline uint32_t nemo_step_lds_bytes(uint32_t T, uint32_t D, uint32_t P, i
 	}
 	if(stdp) {
 		b += NEMO_STDP_LDS_BYTES;      /* STDP tables */
-		b += NEMO_STDP_WORK_CAP * 8u + 64u;  /* STDP work list and its counter */
+		b += NEMO_STDP_WORK_CAP * 8u + 64u;  /* STDP work list and its counters */
+		b += ((P + 1u + 63u) & ~63u) * 4u;   /* prefix of the counts of the targets' cycle */
 	}
 	return b;
 }

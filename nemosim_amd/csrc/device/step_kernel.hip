@@ -371,6 +371,64 @@ __device__ inline void deliver_ranges(uint32_t a0, uint32_t b0, uint32_t a1, uin
 	}
 }
 
+/* Chunks of STDP_CHUNK synapses of the reverse row [r0, r0+len) of the
+ * j-th target of a cycle's list that belong to partition p of P (chunk c
+ * goes to partition (j + c) mod P), appended to the work list. */
+constexpr uint32_t STDP_CHUNK = 64u;
+
+__device__ inline void stdp_list_chunks(uint32_t r0, uint32_t len, uint32_t j, uint32_t p, uint32_t P,
+		uint32_t* cnt, uint2* work)
+{
+	const uint32_t nch = (len + STDP_CHUNK - 1u) / STDP_CHUNK;
+	for(uint32_t ch = (p + P - j % P) % P; ch < nch; ch += P) {
+		const uint32_t slot = atomicAdd(cnt, 1u);
+		if(slot < NEMO_STDP_WORK_CAP) {
+			work[slot] = make_uint2(r0 + ch * STDP_CHUNK, min(STDP_CHUNK, len - ch * STDP_CHUNK));
+		}
+	}
+}
+
+/* Accumulation over a work list of chunks: a wave takes STDP_K chunks at
+ * once, one synapse per lane of each; the reverse records of all of them
+ * are loaded together, then the sources' histories. The same synapse is
+ * accumulated by another partition (on another XCD) when its target appears
+ * at another position of a later cycle's list, so the accumulator is
+ * updated with a device-scope atomic add, not a cached read and write. */
+template<bool IZO>
+__device__ inline void stdp_accumulate(const uint2* work, uint32_t nw, uint32_t wave, uint32_t nwaves,
+		uint32_t lane, const NEMO_GLOBAL uint64_t* rf, const NEMO_GLOBAL uint32_t* rsd,
+		NEMO_GLOBAL int32_t* racc, const StdpParams& sp)
+{
+	constexpr uint32_t STDP_K = IZO ? 8u : 4u;   /* fewer registers left in the generic variant */
+	for(uint32_t w0 = wave * STDP_K; w0 < nw; w0 += nwaves * STDP_K) {   /* wave-uniform */
+		uint32_t r[STDP_K], sd[STDP_K];
+		uint64_t pre[STDP_K];
+		bool ok[STDP_K];
+#pragma unroll
+		for(uint32_t k = 0; k < STDP_K; ++k) {
+			const uint2 wk = (w0 + k < nw) ? work[w0 + k] : make_uint2(0u, 0u);
+			ok[k] = lane < wk.y;
+			r[k] = wk.x + lane;
+		}
+#pragma unroll
+		for(uint32_t k = 0; k < STDP_K; ++k) {
+			/* unconditional loads (entry 0 stands in for absent ones) */
+			sd[k] = rsd[ok[k] ? r[k] : 0u];
+		}
+#pragma unroll
+		for(uint32_t k = 0; k < STDP_K; ++k) {
+			pre[k] = ld_agent(rf + (sd[k] & 0x3ffffffu)) >> ((sd[k] >> 26) + 1u);
+		}
+#pragma unroll
+		for(uint32_t k = 0; k < STDP_K; ++k) {
+			const int32_t wd = ok[k] ? stdp_weight_change(sp, pre[k]) : 0;
+			if(wd != 0) {
+				add_agent(racc + r[k], wd);
+			}
+		}
+	}
+}
+
 /* IZO: every neuron of the network is an Izhikevich neuron, so only its 5
  * parameters and 2 state variables are held in registers. */
 template<bool SAT, bool IZO>
@@ -405,7 +463,8 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 	uint8_t* stdpArea = reinterpret_cast<uint8_t*>(ovf + (SAT ? 2u * D * fwords : 0u));
 	StdpParams* spl = reinterpret_cast<StdpParams*>(stdpArea);
 	uint2* work = reinterpret_cast<uint2*>(stdpArea + NEMO_STDP_LDS_BYTES);   /* [cap] {row index, length} */
-	uint32_t* nwork = reinterpret_cast<uint32_t*>(work + NEMO_STDP_WORK_CAP);
+	uint32_t* nwork = reinterpret_cast<uint32_t*>(work + NEMO_STDP_WORK_CAP);  /* [2], by cycle parity */
+	uint32_t* spref = nwork + 16;   /* prefix of the counts of cycle t-postLen, [P+1] */
 	const bool defer = (nd.flags & NEMO_NET_DEFER) != 0u;
 	const uint32_t offMask = defer ? ((1u << OFF_NEAR_SHIFT) - 1u) : ~0u;
 	const NEMO_GLOBAL uint2* syn = glb(reinterpret_cast<const uint2*>(nd.syn));
@@ -484,6 +543,20 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 		const uint32_t* src = reinterpret_cast<const uint32_t*>(sp);
 		uint32_t* dst = reinterpret_cast<uint32_t*>(spl);
 		for(uint32_t i = tid; i < uint32_t(sizeof(StdpParams) / 4u); i += nthreads) dst[i] = src[i];
+	}
+	/* Early STDP work lists: with postLen >= 2 the work of cycle t is listed
+	 * right after this partition publishes cycle t, from the fired lists and
+	 * reverse-row table of cycle t-postLen, which this partition has seen
+	 * published (its gather of cycle t-1) -- so only the accumulation itself
+	 * waits for cycle t's exchange. */
+	const bool early = stdp && postLen >= 2u;
+	if(early) {
+		if(tid == 0) nwork[nd.cycle0 & 1u] = 0u;
+		if(wave == 0 && nd.cycle0 >= postLen) {
+			const uint64_t tc = nd.cycle0 - postLen;
+			gather_counts(glb(nd.fc) + uint32_t(tc % nd.flRing) * P, P, uint32_t(tc), spref, lane,
+					nd.spinLimit, nd.status, p);
+		}
 	}
 	unsigned long long events = 0;
 	/* deferred delivery: this wave's far ranges [a, b) (delays >= 2) of the
@@ -574,12 +647,13 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 				ovf[(D + slot) * fwords + w] = 0u;
 			}
 		}
+		uint32_t pos = 0;   /* of this neuron in the partition's fired list, if it fired */
 		{
 			uint32_t total;
 			uint32_t base = lds_prefix_waves(wsum, wave, nwaves, &total);
 			if(fired) {
 				const uint64_t below = lane ? (mask & (~0ull >> (64u - lane))) : 0ull;
-				const uint32_t pos = base + uint32_t(__popcll(below));
+				pos = base + uint32_t(__popcll(below));
 				st_agent(glb(nd.fl) + (size_t(flb) * P + p) * T + pos, (uint64_t(uint32_t(t)) << 32) | uint64_t(n));
 				/* Touch every 128-byte line of this neuron's bucket-offset row
 				 * (loads straight into an LDS sink, no registers held), so
@@ -610,6 +684,21 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 			deliver_ranges<SAT>(ka0, kb0, ka1, kb1, kn, nd.glog2, lane, (slot + D - 1u) % D, syn,
 					ringE, ringI, ovf, D, T, fwords);
 			kn = 0;
+		}
+
+		/* ---- STDP work list of this cycle, early (see the prologue): the
+		 * targets are the fired list of cycle t-postLen; chunk c of the j-th
+		 * target goes to partition (j + c) mod P */
+		if(early && t >= postLen) {
+			const uint32_t cb = uint32_t((t - postLen) % nd.flRing);
+			const uint32_t Fc = spref[P];
+			const NEMO_GLOBAL uint64_t* srw = glb(nd.srow) + size_t(cb) * P * T;
+			uint32_t* cnt = nwork + (t & 1u);
+			for(uint32_t j = tid; j < Fc; j += nthreads) {
+				const uint32_t q = partition_of(spref, P, j);
+				const uint64_t rl = ld_agent(srw + size_t(q) * T + (j - spref[q]));
+				stdp_list_chunks(uint32_t(rl), uint32_t(rl >> 32), j, p, P, cnt, work);
+			}
 		}
 
 		/* ---- 3. wait for the network's other partitions */
@@ -707,84 +796,75 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 		 * chunks of STDP_CHUNK, and chunk c of the j-th target of that list
 		 * is processed by partition (j + c) mod P, so every partition takes
 		 * an equal share whichever partition the targets belong to. */
-		if(stdp && t >= postLen) {
-			constexpr uint32_t STDP_CHUNK = 64u;
-			const uint64_t tc = t - postLen;
-			const uint32_t cb = uint32_t(tc % nd.flRing);
-			__syncthreads();   /* pref reuse; delivery done */
-			if(wave == 0) {
-				/* counts of cycle t-postLen, all published long ago */
-				gather_counts(glb(nd.fc) + cb * P, P, uint32_t(tc), pref, lane, nd.spinLimit, nd.status, p);
+		if(stdp) {
+			/* this cycle's fired neurons: their reverse rows, for the lists of
+			 * cycle t+postLen (loaded before the barrier, stored after it) */
+			uint32_t my0 = 0, my1 = 0;
+			if(early && fired) {
+				my0 = glb(nd.rcm_start)[n];
+				my1 = glb(nd.rcm_start)[n + 1];
 			}
-			if(tid == 0) *nwork = 0u;
-			__syncthreads();
-			const uint32_t Fc = pref[P];
-			const NEMO_GLOBAL uint64_t* flc = glb(nd.fl) + size_t(cb) * P * T;
-			const NEMO_GLOBAL uint32_t* rstart = glb(nd.rcm_start);
-			for(uint32_t j = tid; j < Fc; j += nthreads) {
-				const uint32_t q = partition_of(pref, P, j);
-				const NEMO_GLOBAL uint64_t* e = flc + size_t(q) * T + (j - pref[q]);
-				const uint32_t tgt = fired_entry(e, ld_agent(e), uint32_t(tc), nd.spinLimit, nd.status, p, q);
-				const uint32_t r0 = rstart[tgt];
-				const uint32_t len = rstart[tgt + 1] - r0;
-				const uint32_t nch = (len + STDP_CHUNK - 1u) / STDP_CHUNK;
-				for(uint32_t ch = (p + P - j % P) % P; ch < nch; ch += P) {
-					const uint32_t slot = atomicAdd(nwork, 1u);
-					const uint32_t r = r0 + ch * STDP_CHUNK;
-					const uint32_t l = min(STDP_CHUNK, len - ch * STDP_CHUNK);
-					if(slot < NEMO_STDP_WORK_CAP) {
-						work[slot] = make_uint2(r, l);
-					} else {
-						/* list full (bursts on small networks): this thread does it */
-						const NEMO_GLOBAL uint64_t* rf = glb(nd.rfg) + size_t(par) * n_total;
-						for(uint32_t i = 0; i < l; ++i) {
-							const uint32_t sd = glb(nd.rcm_sd)[r + i];
-							const uint64_t pre = ld_agent(rf + (sd & 0x3ffffffu)) >> ((sd >> 26) + 1u);
-							const int32_t wd = stdp_weight_change(*spl, pre);
-							if(wd != 0) {
-								add_agent(glb(nd.rcm_acc) + r + i, wd);
+			__syncthreads();   /* delivery done; the early list is complete; pref reuse */
+			if(early && fired) {
+				st_agent(glb(nd.srow) + (size_t(flb) * P + p) * T + pos, (uint64_t(my1 - my0) << 32) | my0);
+			}
+			const uint32_t listed = early ? nwork[t & 1u] : 0u;
+			if(t >= postLen) {
+				const uint64_t tc = t - postLen;
+				const uint32_t cb = uint32_t(tc % nd.flRing);
+				if(!early || listed > NEMO_STDP_WORK_CAP) {
+					/* the list is built here, with the targets' entries and rows
+					 * read now; chunks beyond the list's capacity are done by
+					 * the thread that finds them */
+					__syncthreads();   /* every wave has read `listed` */
+					if(wave == 0) {
+						gather_counts(glb(nd.fc) + cb * P, P, uint32_t(tc), pref, lane, nd.spinLimit, nd.status, p);
+					}
+					if(tid == 0) nwork[t & 1u] = 0u;
+					__syncthreads();
+					const uint32_t Fc = pref[P];
+					const NEMO_GLOBAL uint64_t* flc = glb(nd.fl) + size_t(cb) * P * T;
+					const NEMO_GLOBAL uint32_t* rstart = glb(nd.rcm_start);
+					const NEMO_GLOBAL uint64_t* rf = glb(nd.rfg) + size_t(par) * n_total;
+					for(uint32_t j = tid; j < Fc; j += nthreads) {
+						const uint32_t q = partition_of(pref, P, j);
+						const NEMO_GLOBAL uint64_t* e = flc + size_t(q) * T + (j - pref[q]);
+						const uint32_t tgt = fired_entry(e, ld_agent(e), uint32_t(tc), nd.spinLimit, nd.status, p, q);
+						const uint32_t r0 = rstart[tgt];
+						const uint32_t len = rstart[tgt + 1] - r0;
+						const uint32_t nch = (len + STDP_CHUNK - 1u) / STDP_CHUNK;
+						for(uint32_t ch = (p + P - j % P) % P; ch < nch; ch += P) {
+							const uint32_t slot = atomicAdd(nwork + (t & 1u), 1u);
+							const uint32_t r = r0 + ch * STDP_CHUNK;
+							const uint32_t l = min(STDP_CHUNK, len - ch * STDP_CHUNK);
+							if(slot < NEMO_STDP_WORK_CAP) {
+								work[slot] = make_uint2(r, l);
+							} else {
+								for(uint32_t i = 0; i < l; ++i) {
+									const uint32_t sd = glb(nd.rcm_sd)[r + i];
+									const uint64_t pre = ld_agent(rf + (sd & 0x3ffffffu)) >> ((sd >> 26) + 1u);
+									const int32_t wd = stdp_weight_change(*spl, pre);
+									if(wd != 0) {
+										add_agent(glb(nd.rcm_acc) + r + i, wd);
+									}
+								}
 							}
 						}
 					}
+					__syncthreads();
 				}
+				const uint32_t nw = min(nwork[t & 1u], NEMO_STDP_WORK_CAP);
+				stdp_accumulate<IZO>(work, nw, wave, nwaves, lane, glb(nd.rfg) + size_t(par) * n_total,
+						glb(nd.rcm_sd), glb(nd.rcm_acc), *spl);
 			}
-			__syncthreads();
-			const uint32_t nw = min(*nwork, NEMO_STDP_WORK_CAP);
-			const NEMO_GLOBAL uint64_t* rf = glb(nd.rfg) + size_t(par) * n_total;
-			const NEMO_GLOBAL uint32_t* rsd = glb(nd.rcm_sd);
-			NEMO_GLOBAL int32_t* racc = glb(nd.rcm_acc);
-			/* a wave takes STDP_K chunks at once, one synapse per lane of each:
-			 * the reverse records of all of them are loaded together, then the
-			 * sources' histories. The same synapse is accumulated by another
-			 * partition (on another XCD) when its target appears at another
-			 * position of a later cycle's list, so the accumulator is updated
-			 * with a device-scope atomic add, not a cached read and write. */
-			constexpr uint32_t STDP_K = IZO ? 8u : 4u;   /* fewer registers left in the generic variant */
-			for(uint32_t w0 = wave * STDP_K; w0 < nw; w0 += nwaves * STDP_K) {   /* wave-uniform */
-				uint32_t r[STDP_K], sd[STDP_K];
-				uint64_t pre[STDP_K];
-				bool ok[STDP_K];
-#pragma unroll
-				for(uint32_t k = 0; k < STDP_K; ++k) {
-					const uint2 wk = (w0 + k < nw) ? work[w0 + k] : make_uint2(0u, 0u);
-					ok[k] = lane < wk.y;
-					r[k] = wk.x + lane;
-				}
-#pragma unroll
-				for(uint32_t k = 0; k < STDP_K; ++k) {
-					/* unconditional loads (entry 0 stands in for absent ones) */
-					sd[k] = rsd[ok[k] ? r[k] : 0u];
-				}
-#pragma unroll
-				for(uint32_t k = 0; k < STDP_K; ++k) {
-					pre[k] = ld_agent(rf + (sd[k] & 0x3ffffffu)) >> ((sd[k] >> 26) + 1u);
-				}
-#pragma unroll
-				for(uint32_t k = 0; k < STDP_K; ++k) {
-					const int32_t wd = ok[k] ? stdp_weight_change(*spl, pre[k]) : 0;
-					if(wd != 0) {
-						add_agent(racc + r[k], wd);
-					}
+			if(early) {
+				/* the next cycle's list starts empty, from the counts of cycle
+				 * t+1-postLen (the list of this cycle is no longer read) */
+				if(tid == 0) nwork[(t + 1u) & 1u] = 0u;
+				if(wave == 0 && t + 1u >= postLen) {
+					const uint64_t tn = t + 1u - postLen;
+					gather_counts(glb(nd.fc) + uint32_t(tn % nd.flRing) * P, P, uint32_t(tn), spref, lane,
+							nd.spinLimit, nd.status, p);
 				}
 			}
 		}

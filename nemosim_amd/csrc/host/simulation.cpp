@@ -461,6 +461,11 @@ Simulation::Simulation(const NetworkImpl& net, const ConfigurationImpl& conf) :
 		d_rcmFwd.upload(rcmFwd.data(), rcmFwd.size() * 4);
 		d_rcmAcc.alloc(std::max<size_t>(rcmSd.size(), 1) * 4);
 		d_stdp.upload(&m_sp, sizeof(m_sp));
+		if(m_fused) {
+			/* reverse row {start, length} of each fired neuron, beside its
+			 * fired-list entry (the next STDP work lists are built from it) */
+			d_srow.alloc(size_t(m_flRing) * m_P * m_T * 8);
+		}
 	}
 	hipStream_t s = static_cast<hipStream_t>(m_stream);
 	if(d_recent.bytes) hipCheck(nemo_dev_fill_u64(d_recent.as<uint64_t>(), 0, m_n, s), "fill");
@@ -494,7 +499,7 @@ Simulation::Simulation(const NetworkImpl& net, const ConfigurationImpl& conf) :
 	devLock.unlock();
 
 	m_deviceBytes = d_model.bytes + d_param.bytes + d_state.bytes + d_rng.bytes + d_recent.bytes
-		+ d_ring.bytes + d_off.bytes + d_syn.bytes + d_fl.bytes + d_fc.bytes + d_rfg.bytes
+		+ d_ring.bytes + d_off.bytes + d_syn.bytes + d_fl.bytes + d_fc.bytes + d_rfg.bytes + d_srow.bytes
 		+ d_rcmStart.bytes + d_rcmSd.bytes + d_rcmFwd.bytes + d_rcmAcc.bytes + d_acc.bytes;
 
 	if(conf.logging) {
@@ -673,6 +678,7 @@ Simulation::prepareLaunch(unsigned nsteps, void* stream, bool clearControl)
 	nd.rcm_fwd = d_rcmFwd.as<uint32_t>();
 	nd.rcm_acc = d_rcmAcc.as<int32_t>();
 	nd.stdp = d_stdp.ptr;
+	nd.srow = d_srow.as<uint64_t>();
 	if(m_hasSchedule) {
 		nd.fs_ptr = d_fsPtr.as<uint32_t>();
 		nd.fs_idx = d_fsIdx.as<uint32_t>();

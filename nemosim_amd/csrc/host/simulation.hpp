@@ -170,7 +170,7 @@ class Simulation
 		/* device data */
 		DevBuf d_model, d_param, d_state, d_rng, d_recent, d_ring;
 		DevBuf d_off, d_syn;
-		DevBuf d_fl, d_fc, d_rfg, d_ctl;       /* ctl: bar, status, events */
+		DevBuf d_fl, d_fc, d_rfg, d_srow, d_ctl;       /* ctl: bar, status, events */
 		DevBuf d_rcmStart, d_rcmSd, d_rcmFwd, d_rcmAcc, d_stdp;
 		DevBuf d_fsPtr, d_fsIdx, d_isPtr, d_isIdx, d_isVal;
 		DevBuf d_raster;
