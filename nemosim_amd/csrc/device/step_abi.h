@@ -49,6 +49,7 @@ typedef struct NetDev {
 	uint32_t spinLimit;  /* barrier spin bound (iterations) */
 	uint32_t glog2;      /* log2 of the lanes that share one fired source in delivery (0..6) */
 	uint32_t flRing;     /* fired lists and counts kept for this many cycles (buffer t mod flRing) */
+	uint32_t stdpCache;  /* STDP chunks whose reverse records are staged in LDS (64 words each) */
 	uint64_t cycle0;     /* cycle count at the start of the launch */
 
 	const uint8_t* model;     /* [n] model id */

@@ -371,33 +371,65 @@ __device__ inline void deliver_ranges(uint32_t a0, uint32_t b0, uint32_t a1, uin
 	}
 }
 
-/* Chunks of STDP_CHUNK synapses of the reverse row [r0, r0+len) of the
- * j-th target of a cycle's list that belong to partition p of P (chunk c
- * goes to partition (j + c) mod P), appended to the work list. */
-constexpr uint32_t STDP_CHUNK = 64u;
+constexpr uint32_t STDP_CHUNK = 64u;   /* synapses per STDP work item (one per lane of a wave) */
 
-__device__ inline void stdp_list_chunks(uint32_t r0, uint32_t len, uint32_t j, uint32_t p, uint32_t P,
-		uint32_t* cnt, uint2* work)
+/* Work list of one cycle, built by one wave for the targets j0 + lane
+ * (< Fc) of the list described by pref: the chunks of STDP_CHUNK synapses
+ * of each target's reverse row [r0, r0+len) that belong to partition p of
+ * P (chunk c of the j-th target goes to partition (j + c) mod P) are
+ * appended to `work`, and the reverse records of the first `cache` chunks
+ * of the list are loaded straight into LDS (sdc, 64 words per chunk: one
+ * wave instruction each, with every lane active). Called by whole waves. */
+__device__ inline void stdp_list_build(uint32_t j0, uint32_t Fc, const uint32_t* pref, uint32_t P, uint32_t p,
+		uint32_t T, const NEMO_GLOBAL uint64_t* srw, const NEMO_GLOBAL uint32_t* rsd, uint32_t lane,
+		uint32_t cache, uint32_t* cnt, uint2* work, uint32_t* sdc)
 {
-	const uint32_t nch = (len + STDP_CHUNK - 1u) / STDP_CHUNK;
-	for(uint32_t ch = (p + P - j % P) % P; ch < nch; ch += P) {
-		const uint32_t slot = atomicAdd(cnt, 1u);
-		if(slot < NEMO_STDP_WORK_CAP) {
-			work[slot] = make_uint2(r0 + ch * STDP_CHUNK, min(STDP_CHUNK, len - ch * STDP_CHUNK));
+	const uint32_t j = j0 + lane;
+	uint32_t r0 = 0, len = 0, ch = 0, nch = 0;
+	if(j < Fc) {
+		const uint32_t q = partition_of(pref, P, j);
+		const uint64_t rl = ld_agent(srw + size_t(q) * T + (j - pref[q]));
+		r0 = uint32_t(rl);
+		len = uint32_t(rl >> 32);
+		nch = (len + STDP_CHUNK - 1u) / STDP_CHUNK;
+		ch = (p + P - j % P) % P;
+	}
+	while(__any(ch < nch)) {   /* wave-uniform */
+		const bool has = ch < nch;
+		const uint32_t r = r0 + ch * STDP_CHUNK;
+		const uint32_t l = has ? min(STDP_CHUNK, len - ch * STDP_CHUNK) : 0u;
+		uint32_t slot = ~0u;
+		if(has) {
+			slot = atomicAdd(cnt, 1u);
+			if(slot < NEMO_STDP_WORK_CAP) {
+				work[slot] = make_uint2(r, l);
+			}
 		}
+		uint64_t m = __ballot(has && slot < cache);
+		while(m) {
+			const uint32_t src = uint32_t(__ffsll((unsigned long long)m) - 1);
+			const uint32_t sl = __builtin_amdgcn_readlane(slot, src);
+			const uint32_t rs = __builtin_amdgcn_readlane(r, src);
+			const uint32_t ls = __builtin_amdgcn_readlane(l, src);   /* >= 1 */
+			__builtin_amdgcn_global_load_lds((NEMO_GLOBAL void*)(rsd + rs + min(lane, ls - 1u)),
+					(NEMO_LDS void*)(sdc + sl * STDP_CHUNK), 4, 0, 0);
+			m &= m - 1u;
+		}
+		ch += P;
 	}
 }
 
 /* Accumulation over a work list of chunks: a wave takes STDP_K chunks at
  * once, one synapse per lane of each; the reverse records of all of them
- * are loaded together, then the sources' histories. The same synapse is
+ * are loaded together (from LDS for the first `ncached` chunks, whose
+ * records were loaded there during delivery), then the sources' histories. The same synapse is
  * accumulated by another partition (on another XCD) when its target appears
  * at another position of a later cycle's list, so the accumulator is
  * updated with a device-scope atomic add, not a cached read and write. */
 template<bool IZO>
 __device__ inline void stdp_accumulate(const uint2* work, uint32_t nw, uint32_t wave, uint32_t nwaves,
 		uint32_t lane, const NEMO_GLOBAL uint64_t* rf, const NEMO_GLOBAL uint32_t* rsd,
-		NEMO_GLOBAL int32_t* racc, const StdpParams& sp)
+		const uint32_t* sdc, uint32_t ncached, NEMO_GLOBAL int32_t* racc, const StdpParams& sp)
 {
 	constexpr uint32_t STDP_K = IZO ? 8u : 4u;   /* fewer registers left in the generic variant */
 	for(uint32_t w0 = wave * STDP_K; w0 < nw; w0 += nwaves * STDP_K) {   /* wave-uniform */
@@ -413,7 +445,7 @@ __device__ inline void stdp_accumulate(const uint2* work, uint32_t nw, uint32_t 
 #pragma unroll
 		for(uint32_t k = 0; k < STDP_K; ++k) {
 			/* unconditional loads (entry 0 stands in for absent ones) */
-			sd[k] = rsd[ok[k] ? r[k] : 0u];
+			sd[k] = (w0 + k < ncached) ? sdc[(w0 + k) * 64u + lane] : rsd[ok[k] ? r[k] : 0u];
 		}
 #pragma unroll
 		for(uint32_t k = 0; k < STDP_K; ++k) {
@@ -465,6 +497,7 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 	uint2* work = reinterpret_cast<uint2*>(stdpArea + NEMO_STDP_LDS_BYTES);   /* [cap] {row index, length} */
 	uint32_t* nwork = reinterpret_cast<uint32_t*>(work + NEMO_STDP_WORK_CAP);  /* [2], by cycle parity */
 	uint32_t* spref = nwork + 16;   /* prefix of the counts of cycle t-postLen, [P+1] */
+	uint32_t* sdcache = spref + ((P + 1u + 63u) & ~63u);   /* [stdpCache][64] reverse records */
 	const bool defer = (nd.flags & NEMO_NET_DEFER) != 0u;
 	const uint32_t offMask = defer ? ((1u << OFF_NEAR_SHIFT) - 1u) : ~0u;
 	const NEMO_GLOBAL uint2* syn = glb(reinterpret_cast<const uint2*>(nd.syn));
@@ -693,11 +726,9 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 			const uint32_t cb = uint32_t((t - postLen) % nd.flRing);
 			const uint32_t Fc = spref[P];
 			const NEMO_GLOBAL uint64_t* srw = glb(nd.srow) + size_t(cb) * P * T;
-			uint32_t* cnt = nwork + (t & 1u);
-			for(uint32_t j = tid; j < Fc; j += nthreads) {
-				const uint32_t q = partition_of(spref, P, j);
-				const uint64_t rl = ld_agent(srw + size_t(q) * T + (j - spref[q]));
-				stdp_list_chunks(uint32_t(rl), uint32_t(rl >> 32), j, p, P, cnt, work);
+			for(uint32_t j0 = wave * 64u; j0 < Fc; j0 += nthreads) {   /* wave-uniform */
+				stdp_list_build(j0, Fc, spref, P, p, T, srw, glb(nd.rcm_sd), lane, nd.stdpCache,
+						nwork + (t & 1u), work, sdcache);
 			}
 		}
 
@@ -808,6 +839,14 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 			if(early && fired) {
 				st_agent(glb(nd.srow) + (size_t(flb) * P + p) * T + pos, (uint64_t(my1 - my0) << 32) | my0);
 			}
+			/* the counts of cycle t+1-postLen, for the next cycle's list, by
+			 * the wave with the fewest chunks (spref is no longer read in
+			 * this cycle) */
+			if(early && wave == nwaves - 1u && t + 1u >= postLen) {
+				const uint64_t tn = t + 1u - postLen;
+				gather_counts(glb(nd.fc) + uint32_t(tn % nd.flRing) * P, P, uint32_t(tn), spref, lane,
+						nd.spinLimit, nd.status, p);
+			}
 			const uint32_t listed = early ? nwork[t & 1u] : 0u;
 			if(t >= postLen) {
 				const uint64_t tc = t - postLen;
@@ -854,18 +893,12 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 					__syncthreads();
 				}
 				const uint32_t nw = min(nwork[t & 1u], NEMO_STDP_WORK_CAP);
+				const uint32_t ncached = (early && listed <= NEMO_STDP_WORK_CAP) ? min(listed, nd.stdpCache) : 0u;
 				stdp_accumulate<IZO>(work, nw, wave, nwaves, lane, glb(nd.rfg) + size_t(par) * n_total,
-						glb(nd.rcm_sd), glb(nd.rcm_acc), *spl);
+						glb(nd.rcm_sd), sdcache, ncached, glb(nd.rcm_acc), *spl);
 			}
-			if(early) {
-				/* the next cycle's list starts empty, from the counts of cycle
-				 * t+1-postLen (the list of this cycle is no longer read) */
-				if(tid == 0) nwork[(t + 1u) & 1u] = 0u;
-				if(wave == 0 && t + 1u >= postLen) {
-					const uint64_t tn = t + 1u - postLen;
-					gather_counts(glb(nd.fc) + uint32_t(tn % nd.flRing) * P, P, uint32_t(tn), spref, lane,
-							nd.spinLimit, nd.status, p);
-				}
+			if(early && tid == 0) {
+				nwork[(t + 1u) & 1u] = 0u;   /* the next cycle's list starts empty */
 			}
 		}
 		__syncthreads();

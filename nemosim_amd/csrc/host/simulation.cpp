@@ -276,6 +276,19 @@ Simulation::Simulation(const NetworkImpl& net, const ConfigurationImpl& conf) :
 			whyNotFused = fmt("network of ", m_n, " neurons needs ", m_P,
 					" co-resident workgroups of ", m_T, " neurons; the device holds ", maxBlocks);
 		}
+		/* STDP: LDS left over stages the reverse records of up to 64 chunks
+		 * (256 bytes each) of every cycle's work list, as long as the
+		 * workgroups stay co-resident */
+		for(uint32_t c = 64; m_stdp && whyNotFused.empty() && c >= 8; c /= 2) {
+			const uint32_t lds = m_lds + c * 256u;
+			if(lds > ldsLimit) continue;
+			hipCheck(nemo_dev_step_max_blocks(m_T, lds, variant(), &maxBlocks), "occupancy query");
+			if(m_P <= maxBlocks) {
+				m_stdpCache = c;
+				m_lds = lds;
+				break;
+			}
+		}
 	}
 	if(!whyNotFused.empty()) {
 		if(conf.stepMode == NEMO_STEP_FUSED) {
@@ -659,6 +672,7 @@ Simulation::prepareLaunch(unsigned nsteps, void* stream, bool clearControl)
 	nd.spinLimit = m_spinLimit;
 	nd.glog2 = m_glog2;
 	nd.flRing = m_flRing;
+	nd.stdpCache = m_stdpCache;
 	nd.cycle0 = m_cycle;
 	nd.model = d_model.as<uint8_t>();
 	nd.param = d_param.as<float>();

@@ -148,6 +148,7 @@ class Simulation
 		bool m_defer = false;
 		/* cycles of fired lists and counts kept on the device */
 		uint32_t m_flRing = 2;
+		uint32_t m_stdpCache = 0;    /* STDP chunks staged in LDS per cycle */
 		/* every neuron is an Izhikevich neuron (NEMO_NET_IZHIKEVICH_ONLY) */
 		bool m_izOnly = false;
 
