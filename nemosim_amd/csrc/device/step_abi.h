@@ -22,6 +22,8 @@ extern "C" {
 #define NEMO_NET_DEFER     8u   /* delay-1 records first in each bucket, their
                                  * count in bits 28..31 of off[]; records of
                                  * delay >= 2 are delivered one cycle later */
+#define NEMO_NET_PREF_RING 32u  /* STDP: prefixes of the fired counts of the last
+                                 * postLen cycles kept in LDS */
 
 /* Bits of the per-network status word */
 #define NEMO_STATUS_BARRIER_TIMEOUT 1u   /* a fired count never arrived */

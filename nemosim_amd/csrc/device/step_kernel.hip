@@ -419,20 +419,24 @@ __device__ inline void stdp_list_build(uint32_t j0, uint32_t Fc, const uint32_t*
 	}
 }
 
-/* Accumulation over a work list of chunks: a wave takes STDP_K chunks at
- * once, one synapse per lane of each; the reverse records of all of them
+/* Accumulation over a work list of chunks: a wave takes the next STDP_K
+ * chunks from the shared counter `next`, one synapse per lane of each; the reverse records of all of them
  * are loaded together (from LDS for the first `ncached` chunks, whose
  * records were loaded there during delivery), then the sources' histories. The same synapse is
  * accumulated by another partition (on another XCD) when its target appears
  * at another position of a later cycle's list, so the accumulator is
  * updated with a device-scope atomic add, not a cached read and write. */
 template<bool IZO>
-__device__ inline void stdp_accumulate(const uint2* work, uint32_t nw, uint32_t wave, uint32_t nwaves,
+__device__ inline void stdp_accumulate(const uint2* work, uint32_t nw, uint32_t* next,
 		uint32_t lane, const NEMO_GLOBAL uint64_t* rf, const NEMO_GLOBAL uint32_t* rsd,
 		const uint32_t* sdc, uint32_t ncached, NEMO_GLOBAL int32_t* racc, const StdpParams& sp)
 {
 	constexpr uint32_t STDP_K = IZO ? 8u : 4u;   /* fewer registers left in the generic variant */
-	for(uint32_t w0 = wave * STDP_K; w0 < nw; w0 += nwaves * STDP_K) {   /* wave-uniform */
+	for(;;) {
+		uint32_t w0 = 0;
+		if(lane == 0) w0 = atomicAdd(next, STDP_K);
+		w0 = __builtin_amdgcn_readfirstlane(w0);   /* wave-uniform */
+		if(w0 >= nw) break;
 		uint32_t r[STDP_K], sd[STDP_K];
 		uint64_t pre[STDP_K];
 		bool ok[STDP_K];
@@ -498,6 +502,10 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 	uint32_t* nwork = reinterpret_cast<uint32_t*>(work + NEMO_STDP_WORK_CAP);  /* [2], by cycle parity */
 	uint32_t* spref = nwork + 16;   /* prefix of the counts of cycle t-postLen, [P+1] */
 	uint32_t* sdcache = spref + ((P + 1u + 63u) & ~63u);   /* [stdpCache][64] reverse records */
+	/* with NEMO_NET_PREF_RING: the prefixes of the counts of the last postLen
+	 * cycles, [postLen][(P+1) rounded up to 64], slot t mod postLen */
+	uint32_t* pringArea = sdcache + nd.stdpCache * 64u;
+	const uint32_t PW = (P + 1u + 63u) & ~63u;
 	const bool defer = (nd.flags & NEMO_NET_DEFER) != 0u;
 	const uint32_t offMask = defer ? ((1u << OFF_NEAR_SHIFT) - 1u) : ~0u;
 	const NEMO_GLOBAL uint2* syn = glb(reinterpret_cast<const uint2*>(nd.syn));
@@ -583,13 +591,26 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 	 * published (its gather of cycle t-1) -- so only the accumulation itself
 	 * waits for cycle t's exchange. */
 	const bool early = stdp && postLen >= 2u;
-	if(early) {
-		if(tid == 0) nwork[nd.cycle0 & 1u] = 0u;
-		if(wave == 0 && nd.cycle0 >= postLen) {
-			const uint64_t tc = nd.cycle0 - postLen;
-			gather_counts(glb(nd.fc) + uint32_t(tc % nd.flRing) * P, P, uint32_t(tc), spref, lane,
-					nd.spinLimit, nd.status, p);
+	const bool pring = early && (nd.flags & NEMO_NET_PREF_RING);
+	uint32_t rslot = pring ? uint32_t(nd.cycle0 % postLen) : 0u;   /* t mod postLen */
+	if(stdp && tid == 0) {
+		nwork[0] = nwork[1] = 0u;   /* list lengths, by cycle parity */
+		nwork[2] = nwork[3] = 0u;   /* chunks taken from them */
+	}
+	if(pring) {
+		/* the prefixes of the launch's first postLen cycles' lists, one
+		 * earlier cycle per wave at a time */
+		for(uint32_t i = wave; i < postLen; i += nwaves) {
+			if(nd.cycle0 >= postLen - i) {
+				const uint64_t tc = nd.cycle0 - postLen + i;
+				gather_counts(glb(nd.fc) + uint32_t(tc % nd.flRing) * P, P, uint32_t(tc),
+						pringArea + uint32_t(tc % postLen) * PW, lane, nd.spinLimit, nd.status, p);
+			}
 		}
+	} else if(early && wave == 0 && nd.cycle0 >= postLen) {
+		const uint64_t tc = nd.cycle0 - postLen;
+		gather_counts(glb(nd.fc) + uint32_t(tc % nd.flRing) * P, P, uint32_t(tc), spref, lane,
+				nd.spinLimit, nd.status, p);
 	}
 	unsigned long long events = 0;
 	/* deferred delivery: this wave's far ranges [a, b) (delays >= 2) of the
@@ -680,13 +701,12 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 				ovf[(D + slot) * fwords + w] = 0u;
 			}
 		}
-		uint32_t pos = 0;   /* of this neuron in the partition's fired list, if it fired */
 		{
 			uint32_t total;
 			uint32_t base = lds_prefix_waves(wsum, wave, nwaves, &total);
+			const uint64_t below = lane ? (mask & (~0ull >> (64u - lane))) : 0ull;
+			const uint32_t pos = base + uint32_t(__popcll(below));   /* in the partition's fired list */
 			if(fired) {
-				const uint64_t below = lane ? (mask & (~0ull >> (64u - lane))) : 0ull;
-				pos = base + uint32_t(__popcll(below));
 				st_agent(glb(nd.fl) + (size_t(flb) * P + p) * T + pos, (uint64_t(uint32_t(t)) << 32) | uint64_t(n));
 				/* Touch every 128-byte line of this neuron's bucket-offset row
 				 * (loads straight into an LDS sink, no registers held), so
@@ -701,10 +721,21 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 			 * readers validate each entry and need no ordering against the
 			 * count; the STDP firing history does not, so with STDP every
 			 * wave's stores are drained first (s_waitcnt; __syncthreads alone
-			 * waits only for LDS traffic). */
+			 * waits only for LDS traffic). The reverse row of each fired
+			 * neuron (the STDP lists of cycle t+postLen are built from it) is
+			 * loaded meanwhile and stored after the drain: the publish of
+			 * cycle t+1 drains it. */
 			if(stdp) {
+				uint32_t my0 = 0, my1 = 0;
+				if(early && fired) {
+					my0 = glb(nd.rcm_start)[n];
+					my1 = glb(nd.rcm_start)[n + 1];
+				}
 				__builtin_amdgcn_s_waitcnt(0);
 				__syncthreads();
+				if(early && fired) {
+					st_agent(glb(nd.srow) + (size_t(flb) * P + p) * T + pos, (uint64_t(my1 - my0) << 32) | my0);
+				}
 			}
 			if(tid == 0) {
 				st_agent(glb(nd.fc) + flb * P + p, (uint64_t(uint32_t(t)) << 32) | uint64_t(total));
@@ -724,12 +755,16 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 		 * target goes to partition (j + c) mod P */
 		if(early && t >= postLen) {
 			const uint32_t cb = uint32_t((t - postLen) % nd.flRing);
-			const uint32_t Fc = spref[P];
+			const uint32_t* sp = pring ? pringArea + rslot * PW : spref;
+			const uint32_t Fc = sp[P];
 			const NEMO_GLOBAL uint64_t* srw = glb(nd.srow) + size_t(cb) * P * T;
 			for(uint32_t j0 = wave * 64u; j0 < Fc; j0 += nthreads) {   /* wave-uniform */
-				stdp_list_build(j0, Fc, spref, P, p, T, srw, glb(nd.rcm_sd), lane, nd.stdpCache,
+				stdp_list_build(j0, Fc, sp, P, p, T, srw, glb(nd.rcm_sd), lane, nd.stdpCache,
 						nwork + (t & 1u), work, sdcache);
 			}
+			/* the staged records land before the exchange barrier: after it,
+			 * any wave may read them (wave 0's gather waits for them anyway) */
+			if(wave != 0) __builtin_amdgcn_s_waitcnt(0);
 		}
 
 		/* ---- 3. wait for the network's other partitions */
@@ -737,6 +772,13 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 			gather_counts(glb(nd.fc) + flb * P, P, uint32_t(t), pref, lane, nd.spinLimit, nd.status, p);
 		}
 		__syncthreads();
+
+		if(pring) {
+			/* kept for the list of cycle t+postLen (slot t mod postLen, whose
+			 * cycle t-postLen list was built above) */
+			for(uint32_t i = tid; i <= P; i += nthreads) pringArea[rslot * PW + i] = pref[i];
+			rslot = rslot + 1u == postLen ? 0u : rslot + 1u;
+		}
 
 		/* ---- 4. delivery of this cycle's spikes into the ring */
 		{
@@ -826,79 +868,74 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 		 * of postLen+2 cycles); their plastic in-synapse rows are cut into
 		 * chunks of STDP_CHUNK, and chunk c of the j-th target of that list
 		 * is processed by partition (j + c) mod P, so every partition takes
-		 * an equal share whichever partition the targets belong to. */
+		 * an equal share whichever partition the targets belong to. With the
+		 * list built early, each wave starts on it as soon as its own
+		 * delivery is done, taking STDP_K chunks at a time from a shared
+		 * counter (the waves that finish delivery first take more). */
 		if(stdp) {
-			/* this cycle's fired neurons: their reverse rows, for the lists of
-			 * cycle t+postLen (loaded before the barrier, stored after it) */
-			uint32_t my0 = 0, my1 = 0;
-			if(early && fired) {
-				my0 = glb(nd.rcm_start)[n];
-				my1 = glb(nd.rcm_start)[n + 1];
-			}
-			__syncthreads();   /* delivery done; the early list is complete; pref reuse */
-			if(early && fired) {
-				st_agent(glb(nd.srow) + (size_t(flb) * P + p) * T + pos, (uint64_t(my1 - my0) << 32) | my0);
-			}
-			/* the counts of cycle t+1-postLen, for the next cycle's list, by
-			 * the wave with the fewest chunks (spref is no longer read in
-			 * this cycle) */
-			if(early && wave == nwaves - 1u && t + 1u >= postLen) {
-				const uint64_t tn = t + 1u - postLen;
-				gather_counts(glb(nd.fc) + uint32_t(tn % nd.flRing) * P, P, uint32_t(tn), spref, lane,
-						nd.spinLimit, nd.status, p);
-			}
-			const uint32_t listed = early ? nwork[t & 1u] : 0u;
-			if(t >= postLen) {
+			const uint32_t listed = early ? nwork[t & 1u] : 0u;   /* final since the exchange barrier */
+			const bool late = t >= postLen && (!early || listed > NEMO_STDP_WORK_CAP);
+			if(late) {
+				/* the list is built here, with the targets' entries and rows
+				 * read now; chunks beyond the list's capacity are done by the
+				 * thread that finds them */
 				const uint64_t tc = t - postLen;
 				const uint32_t cb = uint32_t(tc % nd.flRing);
-				if(!early || listed > NEMO_STDP_WORK_CAP) {
-					/* the list is built here, with the targets' entries and rows
-					 * read now; chunks beyond the list's capacity are done by
-					 * the thread that finds them */
-					__syncthreads();   /* every wave has read `listed` */
-					if(wave == 0) {
-						gather_counts(glb(nd.fc) + cb * P, P, uint32_t(tc), pref, lane, nd.spinLimit, nd.status, p);
-					}
-					if(tid == 0) nwork[t & 1u] = 0u;
-					__syncthreads();
-					const uint32_t Fc = pref[P];
-					const NEMO_GLOBAL uint64_t* flc = glb(nd.fl) + size_t(cb) * P * T;
-					const NEMO_GLOBAL uint32_t* rstart = glb(nd.rcm_start);
-					const NEMO_GLOBAL uint64_t* rf = glb(nd.rfg) + size_t(par) * n_total;
-					for(uint32_t j = tid; j < Fc; j += nthreads) {
-						const uint32_t q = partition_of(pref, P, j);
-						const NEMO_GLOBAL uint64_t* e = flc + size_t(q) * T + (j - pref[q]);
-						const uint32_t tgt = fired_entry(e, ld_agent(e), uint32_t(tc), nd.spinLimit, nd.status, p, q);
-						const uint32_t r0 = rstart[tgt];
-						const uint32_t len = rstart[tgt + 1] - r0;
-						const uint32_t nch = (len + STDP_CHUNK - 1u) / STDP_CHUNK;
-						for(uint32_t ch = (p + P - j % P) % P; ch < nch; ch += P) {
-							const uint32_t slot = atomicAdd(nwork + (t & 1u), 1u);
-							const uint32_t r = r0 + ch * STDP_CHUNK;
-							const uint32_t l = min(STDP_CHUNK, len - ch * STDP_CHUNK);
-							if(slot < NEMO_STDP_WORK_CAP) {
-								work[slot] = make_uint2(r, l);
-							} else {
-								for(uint32_t i = 0; i < l; ++i) {
-									const uint32_t sd = glb(nd.rcm_sd)[r + i];
-									const uint64_t pre = ld_agent(rf + (sd & 0x3ffffffu)) >> ((sd >> 26) + 1u);
-									const int32_t wd = stdp_weight_change(*spl, pre);
-									if(wd != 0) {
-										add_agent(glb(nd.rcm_acc) + r + i, wd);
-									}
+				__syncthreads();   /* delivery (which reads pref) done; every wave has read `listed` */
+				if(wave == 0) {
+					gather_counts(glb(nd.fc) + cb * P, P, uint32_t(tc), pref, lane, nd.spinLimit, nd.status, p);
+				}
+				if(tid == 0) nwork[t & 1u] = 0u;
+				__syncthreads();
+				const uint32_t Fc = pref[P];
+				const NEMO_GLOBAL uint64_t* flc = glb(nd.fl) + size_t(cb) * P * T;
+				const NEMO_GLOBAL uint32_t* rstart = glb(nd.rcm_start);
+				const NEMO_GLOBAL uint64_t* rf = glb(nd.rfg) + size_t(par) * n_total;
+				for(uint32_t j = tid; j < Fc; j += nthreads) {
+					const uint32_t q = partition_of(pref, P, j);
+					const NEMO_GLOBAL uint64_t* e = flc + size_t(q) * T + (j - pref[q]);
+					const uint32_t tgt = fired_entry(e, ld_agent(e), uint32_t(tc), nd.spinLimit, nd.status, p, q);
+					const uint32_t r0 = rstart[tgt];
+					const uint32_t len = rstart[tgt + 1] - r0;
+					const uint32_t nch = (len + STDP_CHUNK - 1u) / STDP_CHUNK;
+					for(uint32_t ch = (p + P - j % P) % P; ch < nch; ch += P) {
+						const uint32_t slot = atomicAdd(nwork + (t & 1u), 1u);
+						const uint32_t r = r0 + ch * STDP_CHUNK;
+						const uint32_t l = min(STDP_CHUNK, len - ch * STDP_CHUNK);
+						if(slot < NEMO_STDP_WORK_CAP) {
+							work[slot] = make_uint2(r, l);
+						} else {
+							for(uint32_t i = 0; i < l; ++i) {
+								const uint32_t sd = glb(nd.rcm_sd)[r + i];
+								const uint64_t pre = ld_agent(rf + (sd & 0x3ffffffu)) >> ((sd >> 26) + 1u);
+								const int32_t wd = stdp_weight_change(*spl, pre);
+								if(wd != 0) {
+									add_agent(glb(nd.rcm_acc) + r + i, wd);
 								}
 							}
 						}
 					}
-					__syncthreads();
 				}
+				__syncthreads();
+			}
+			/* without the prefix ring: the counts of cycle t+1-postLen, for
+			 * the next cycle's list, by the last wave (spref is no longer
+			 * read in this cycle) */
+			if(early && !pring && wave == nwaves - 1u && t + 1u >= postLen) {
+				const uint64_t tn = t + 1u - postLen;
+				gather_counts(glb(nd.fc) + uint32_t(tn % nd.flRing) * P, P, uint32_t(tn), spref, lane,
+						nd.spinLimit, nd.status, p);
+			}
+			if(t >= postLen) {
 				const uint32_t nw = min(nwork[t & 1u], NEMO_STDP_WORK_CAP);
-				const uint32_t ncached = (early && listed <= NEMO_STDP_WORK_CAP) ? min(listed, nd.stdpCache) : 0u;
-				stdp_accumulate<IZO>(work, nw, wave, nwaves, lane, glb(nd.rfg) + size_t(par) * n_total,
+				const uint32_t ncached = late ? 0u : min(listed, nd.stdpCache);
+				stdp_accumulate<IZO>(work, nw, nwork + 2u + (t & 1u), lane, glb(nd.rfg) + size_t(par) * n_total,
 						glb(nd.rcm_sd), sdcache, ncached, glb(nd.rcm_acc), *spl);
 			}
-			if(early && tid == 0) {
-				nwork[(t + 1u) & 1u] = 0u;   /* the next cycle's list starts empty */
+			if(tid == 0) {
+				/* the next cycle's list and chunk counter start empty */
+				nwork[(t + 1u) & 1u] = 0u;
+				nwork[2u + ((t + 1u) & 1u)] = 0u;
 			}
 		}
 		__syncthreads();

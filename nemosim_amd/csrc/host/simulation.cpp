@@ -276,16 +276,28 @@ Simulation::Simulation(const NetworkImpl& net, const ConfigurationImpl& conf) :
 			whyNotFused = fmt("network of ", m_n, " neurons needs ", m_P,
 					" co-resident workgroups of ", m_T, " neurons; the device holds ", maxBlocks);
 		}
-		/* STDP: LDS left over stages the reverse records of up to 64 chunks
-		 * (256 bytes each) of every cycle's work list, as long as the
-		 * workgroups stay co-resident */
-		for(uint32_t c = 64; m_stdp && whyNotFused.empty() && c >= 8; c /= 2) {
-			const uint32_t lds = m_lds + c * 256u;
-			if(lds > ldsLimit) continue;
+		/* STDP, with LDS left over while the workgroups stay co-resident:
+		 * first the prefixes of the fired counts of the last postLen cycles
+		 * (the work lists' targets; otherwise gathered again each cycle),
+		 * then the reverse records of up to 64 chunks (256 bytes each) of
+		 * every cycle's work list */
+		auto fits = [&](uint32_t lds) {
+			if(lds > ldsLimit) return false;
 			hipCheck(nemo_dev_step_max_blocks(m_T, lds, variant(), &maxBlocks), "occupancy query");
-			if(m_P <= maxBlocks) {
+			return m_P <= maxBlocks;
+		};
+		const uint32_t postLen = uint32_t(m_sp.postLen);
+		if(m_stdp && whyNotFused.empty() && postLen >= 2) {
+			const uint32_t ring = postLen * ((m_P + 1u + 63u) & ~63u) * 4u;
+			if(fits(m_lds + ring)) {
+				m_prefRing = true;
+				m_lds += ring;
+			}
+		}
+		for(uint32_t c = 64; m_stdp && whyNotFused.empty() && c >= 8; c /= 2) {
+			if(fits(m_lds + c * 256u)) {
 				m_stdpCache = c;
-				m_lds = lds;
+				m_lds += c * 256u;
 				break;
 			}
 		}
@@ -667,7 +679,7 @@ Simulation::prepareLaunch(unsigned nsteps, void* stream, bool clearControl)
 	nd.words = m_words;
 	nd.flags = (m_stdp ? NEMO_NET_STDP : 0u) | (m_saturate ? NEMO_NET_SATURATE : 0u)
 		| (m_replay ? NEMO_NET_REPLAY : 0u) | (m_defer ? NEMO_NET_DEFER : 0u)
-		| (m_izOnly ? NEMO_NET_IZHIKEVICH_ONLY : 0u);
+		| (m_izOnly ? NEMO_NET_IZHIKEVICH_ONLY : 0u) | (m_prefRing ? NEMO_NET_PREF_RING : 0u);
 	nd.nsteps = nsteps;
 	nd.spinLimit = m_spinLimit;
 	nd.glog2 = m_glog2;

@@ -149,6 +149,7 @@ class Simulation
 		/* cycles of fired lists and counts kept on the device */
 		uint32_t m_flRing = 2;
 		uint32_t m_stdpCache = 0;    /* STDP chunks staged in LDS per cycle */
+		bool m_prefRing = false;     /* STDP list prefixes of postLen cycles kept in LDS */
 		/* every neuron is an Izhikevich neuron (NEMO_NET_IZHIKEVICH_ONLY) */
 		bool m_izOnly = false;
 
