@@ -129,12 +129,15 @@ int nemo_dev_fill_u64(uint64_t* p, uint64_t value, size_t n, void* stream);
 int nemo_dev_eval_libm(int kind, const float* in_dev, float* out_dev, size_t n, void* stream);
 int nemo_dev_nrand_seq(const uint32_t* state4_dev, float* out_dev, size_t n, void* stream);
 
-/* LDS bytes the step kernel needs for partitions of T neurons, ring depth D,
- * P partitions. */
 /* Bytes of LDS the STDP tables take (the StdpParams of models.h, rounded) */
 #define NEMO_STDP_LDS_BYTES 576u
 /* Entries of a workgroup's per-cycle STDP work list (64-synapse chunks) */
 #define NEMO_STDP_WORK_CAP 1024u
+
+/* LDS bytes the step kernel needs for partitions of T neurons, ring depth D,
+ * P partitions. Optional STDP areas follow, in this order, when the host
+ * finds room for them: stdpCache x 256 bytes of staged reverse records, then
+ * with NEMO_NET_PREF_RING postLen x ((P+1) rounded up to 64) x 4 bytes. */
 
 static inline uint32_t nemo_step_lds_bytes(uint32_t T, uint32_t D, uint32_t P, int saturate, int stdp)
 {
