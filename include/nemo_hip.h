@@ -54,6 +54,13 @@ nemo_status_t nemo_set_partition_size(nemo_configuration_t conf, unsigned neuron
  * in NeMo; a tuning knob of the fused step kernel for the autotuner. */
 nemo_status_t nemo_set_delivery_lanes(nemo_configuration_t conf, unsigned lanes);
 
+/* STDP work lists in LDS (default on): with it the fused kernel keeps the
+ * fired-count prefixes of the last postLen cycles and the reverse records of
+ * each cycle's first STDP chunks in LDS, as far as LDS allows; off, it reads
+ * both from device memory every cycle. Same results either way. No
+ * counterpart in NeMo; a tuning knob of the fused step kernel. */
+nemo_status_t nemo_set_stdp_lds_staging(nemo_configuration_t conf, int enable);
+
 /* Which step implementation runs a simulation:
  *   NEMO_STEP_AUTO    the fused persistent kernel when every neuron type is a
  *                     built-in model and the network fits it; else plugins
@@ -130,6 +137,8 @@ typedef struct nemo_hip_info {
 	unsigned fused;              /* 1: fused persistent step kernel; 0: plugin step */
 	unsigned deferred;           /* 1: records of delay >= 2 delivered one cycle late */
 	unsigned lanes_per_source;   /* lanes sharing one fired source in delivery */
+	unsigned stdp_staged_chunks; /* STDP chunks whose reverse records are staged in LDS per cycle */
+	unsigned stdp_prefix_ring;   /* 1: fired-count prefixes of the last postLen cycles kept in LDS */
 } nemo_hip_info_t;
 
 nemo_status_t nemo_hip_get_info(nemo_simulation_t sim, nemo_hip_info_t* info);

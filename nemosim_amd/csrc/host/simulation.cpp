@@ -287,14 +287,14 @@ Simulation::Simulation(const NetworkImpl& net, const ConfigurationImpl& conf) :
 			return m_P <= maxBlocks;
 		};
 		const uint32_t postLen = uint32_t(m_sp.postLen);
-		if(m_stdp && whyNotFused.empty() && postLen >= 2) {
+		if(m_stdp && conf.stdpLdsStaging && whyNotFused.empty() && postLen >= 2) {
 			const uint32_t ring = postLen * ((m_P + 1u + 63u) & ~63u) * 4u;
 			if(fits(m_lds + ring)) {
 				m_prefRing = true;
 				m_lds += ring;
 			}
 		}
-		for(uint32_t c = 64; m_stdp && whyNotFused.empty() && c >= 8; c /= 2) {
+		for(uint32_t c = 64; m_stdp && conf.stdpLdsStaging && whyNotFused.empty() && c >= 8; c /= 2) {
 			if(fits(m_lds + c * 256u)) {
 				m_stdpCache = c;
 				m_lds += c * 256u;
@@ -970,6 +970,8 @@ Simulation::info(nemo_hip_info_t* out) const
 	out->fused = m_fused ? 1u : 0u;
 	out->deferred = m_defer ? 1u : 0u;
 	out->lanes_per_source = m_fused ? (1u << m_glog2) : 0u;
+	out->stdp_staged_chunks = m_stdpCache;
+	out->stdp_prefix_ring = m_prefRing ? 1u : 0u;
 }
 
 /* src/nemo/ConnectivityMatrix.cpp:241-294 */

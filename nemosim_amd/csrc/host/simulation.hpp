@@ -28,6 +28,7 @@ struct ConfigurationImpl {
 	bool logging = false;
 	unsigned partitionSize = 0;   /* 0 = automatic */
 	unsigned deliveryLanes = 0;   /* lanes per fired source in delivery; 0 = automatic */
+	bool stdpLdsStaging = true;   /* STDP count prefixes and reverse records staged in LDS */
 	int stepMode = NEMO_STEP_AUTO;
 	std::string description;
 

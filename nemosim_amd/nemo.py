@@ -96,6 +96,10 @@ class Configuration(object):
         """Lanes sharing one fired source in delivery (power of two <= 64; 0 = automatic)."""
         _check(self._lib.nemo_set_delivery_lanes(self._h, int(lanes)))
 
+    def set_stdp_lds_staging(self, enable):
+        """STDP count prefixes and reverse records staged in LDS (default on; same results)."""
+        _check(self._lib.nemo_set_stdp_lds_staging(self._h, 1 if enable else 0))
+
     def set_step_mode(self, mode):
         """'auto', 'fused' or 'plugins' (see nemo_set_step_mode)."""
         _check(self._lib.nemo_set_step_mode(self._h, STEP_MODES[mode]))

@@ -184,6 +184,52 @@ def test_stdp_parity(nemo_mod, oracle_mod):
     assert np.any(ref_w != np.array([int(np.trunc(np.float64(w) * (1 << 20))) for w in spec.weights], np.int64))
 
 
+def _compare_weights(sim, orc, spec, nsample, seed=0):
+    ref_w = orc.weights_fx(spec.n, spec.nsyn)
+    sample = np.random.default_rng(seed).integers(0, spec.nsyn, size=nsample)
+    for k in sample:
+        src = int(np.searchsorted(spec.rowptr, k, side="right") - 1)
+        sid = (src << 32) | int(k - spec.rowptr[src])
+        got = sim.get_synapse_weight(sid)
+        exp = np.float32(np.float32(ref_w[k]) / np.float32(1 << 20))
+        assert got == exp, (k, got, exp)
+
+
+@pytest.mark.parametrize("staging", [True, False])
+def test_stdp_work_lists(nemo_mod, oracle_mod, staging):
+    """STDP work lists with and without LDS staging (count-prefix ring and
+    reverse records in LDS), in one-cycle launches (each launch rebuilds
+    the ring from the counts of the last postLen cycles) and with bursts of
+    3000 forced neurons: postLen cycles later each of the 32 partitions has
+    about 1200 chunks of 64 synapses to accumulate, more than its list
+    holds (1024), so the list is rebuilt after delivery and the excess is
+    accumulated inline. Rasters and weights bit-exact."""
+    n = 4000
+    spec = examples.izhikevich_80_20(n, 1000, seed=21, stdp=True)
+    pre, post = examples.stdp_window()
+    net = spec.to_nemo(nemo_mod)
+    conf = nemo_mod.Configuration()
+    conf.set_stdp_function(pre, post, min_e=0.0, max_e=1.0, min_i=0.0, max_i=-1.0)
+    conf.set_partition_size(128)
+    conf.set_stdp_lds_staging(staging)
+    sim = nemo_mod.Simulation(net, conf)
+    info = sim.info()
+    assert info["partitions"] == 32
+    assert info["stdp_prefix_ring"] == (1 if staging else 0), info
+    assert (info["stdp_staged_chunks"] > 0) == staging, info
+    orc = oracle_mod.from_spec(spec, stdp=(pre, post, 0.0, 1.0, 0.0, -1.0))
+    _compare_run(sim, orc, 50)
+    rng = np.random.default_rng(8)
+    for t in range(60):
+        fs = sorted(rng.choice(n, size=3000, replace=False).tolist()) if t % 30 == 0 else []
+        assert sim.step(fs) == orc.step(fs), t
+    _compare_run(sim, orc, 100)
+    sim.apply_stdp(1.0)
+    orc.apply_stdp(1.0)
+    _compare_run(sim, orc, 50)
+    _compare_weights(sim, orc, spec, 3000)
+
+
 def test_synapse_queries(nemo_mod, oracle_mod):
     spec = examples.izhikevich_80_20(300, 20, seed=2)
     sim = _hip_sim(nemo_mod, spec)
