@@ -61,6 +61,13 @@ nemo_status_t nemo_set_delivery_lanes(nemo_configuration_t conf, unsigned lanes)
  * counterpart in NeMo; a tuning knob of the fused step kernel. */
 nemo_status_t nemo_set_stdp_lds_staging(nemo_configuration_t conf, int enable);
 
+/* Synapse structures (forward matrix, reverse matrix, query tables) built
+ * on the device (default on) or on the host. Same results either way; a
+ * network with an invalid synapse is always checked on the host, which
+ * raises NeMo's error. No counterpart in NeMo, whose construction is on the
+ * host (src/nemo/ConnectivityMatrix.cpp:98-161). */
+nemo_status_t nemo_set_device_construction(nemo_configuration_t conf, int enable);
+
 /* Which step implementation runs a simulation:
  *   NEMO_STEP_AUTO    the fused persistent kernel when every neuron type is a
  *                     built-in model and the network fits it; else plugins
@@ -139,6 +146,7 @@ typedef struct nemo_hip_info {
 	unsigned lanes_per_source;   /* lanes sharing one fired source in delivery */
 	unsigned stdp_staged_chunks; /* STDP chunks whose reverse records are staged in LDS per cycle */
 	unsigned stdp_prefix_ring;   /* 1: fired-count prefixes of the last postLen cycles kept in LDS */
+	unsigned device_built;       /* 1: synapse structures were built on the device */
 } nemo_hip_info_t;
 
 nemo_status_t nemo_hip_get_info(nemo_simulation_t sim, nemo_hip_info_t* info);

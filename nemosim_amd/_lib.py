@@ -50,6 +50,7 @@ class InfoT(ctypes.Structure):
         ("lanes_per_source", ctypes.c_uint),
         ("stdp_staged_chunks", ctypes.c_uint),
         ("stdp_prefix_ring", ctypes.c_uint),
+        ("device_built", ctypes.c_uint),
     ]
 
 
@@ -121,6 +122,7 @@ SIGNATURES = {
     "nemo_set_partition_size": (ctypes.c_int, [c_void_p, ctypes.c_uint]),
     "nemo_set_delivery_lanes": (ctypes.c_int, [c_void_p, ctypes.c_uint]),
     "nemo_set_stdp_lds_staging": (ctypes.c_int, [c_void_p, ctypes.c_int]),
+    "nemo_set_device_construction": (ctypes.c_int, [c_void_p, ctypes.c_int]),
     "nemo_set_step_mode": (ctypes.c_int, [c_void_p, ctypes.c_int]),
     "nemo_hip_run": (ctypes.c_int, [c_void_p, ctypes.c_uint, ctypes.POINTER(ScheduleT), c_ull_p]),
     "nemo_hip_read_fired": (ctypes.c_int, [c_void_p, c_uint_p, c_uint_p, ctypes.c_size_t, c_size_p]),

@@ -158,6 +158,8 @@ nemo_set_delivery_lanes(nemo_configuration_t c, unsigned lanes)
 
 nemo_status_t nemo_set_stdp_lds_staging(nemo_configuration_t c, int enable) { RETURN_CALL(c->impl.stdpLdsStaging = enable != 0); }
 
+nemo_status_t nemo_set_device_construction(nemo_configuration_t c, int enable) { RETURN_CALL(c->impl.deviceConstruction = enable != 0); }
+
 nemo_status_t
 nemo_set_step_mode(nemo_configuration_t c, int mode)
 {

@@ -191,42 +191,88 @@ Simulation::Simulation(const NetworkImpl& net, const ConfigurationImpl& conf) :
 	hipCheck(hipDeviceGetAttribute(&ldsMax, hipDeviceAttributeMaxSharedMemoryPerBlock, dev), "hipDeviceGetAttribute");
 	const uint32_t ldsLimit = std::min<uint32_t>(LDS_LIMIT, uint32_t(ldsMax));
 
-	/* ---- synapse validation and saturation bound */
+	/* ---- synapse validation and saturation bound. On the device when the
+	 * configuration allows (each synapse mapped to local ids, its weight
+	 * converted, the bounds summed per target); the host loop does the same
+	 * and raises the errors, so it runs whenever the device finds a synapse
+	 * it must look at. */
 	const std::vector<unsigned>& S = net.src();
 	const std::vector<unsigned>& Tg = net.tgt();
 	const std::vector<unsigned char>& Dl = net.delay();
 	const std::vector<float>& W = net.weight();
 	const std::vector<unsigned char>& Pl = net.plastic();
 	m_nsyn = S.size();
-	std::vector<uint32_t> lsrc(m_nsyn), ltgt(m_nsyn);
-	std::vector<int32_t> wfx(m_nsyn);
-	std::vector<int64_t> boundE(m_n, 0), boundI(m_n, 0);
-	for(size_t i = 0; i < m_nsyn; ++i) {
-		unsigned s = S[i], t = Tg[i];
-		uint32_t ls = (s >= m_minG && size_t(s - m_minG) < m_dense.size()) ? m_dense[s - m_minG] : 0u;
-		if(!ls) {
-			throw exception(NEMO_INVALID_INPUT, fmt("Invalid synapse source neuron ", s));
+	hipStream_t bs = static_cast<hipStream_t>(m_stream);
+	std::unique_lock<std::recursive_mutex> buildLock(deviceMutex(), std::defer_lock);
+	DevBuf dS, dTg, dW, dPl, dDl, dLsrc, dLtgt, dWfx;
+	bool onDevice = conf.deviceConstruction && m_nsyn > 0 && m_nsyn < (size_t(1) << 32) && m_n > 0;
+	uint32_t bflags = 0;
+	if(onDevice) {
+		buildLock.lock();
+		DevBuf dDense, dBound, dFlags;
+		dS.upload(S.data(), m_nsyn * 4);
+		dTg.upload(Tg.data(), m_nsyn * 4);
+		dW.upload(W.data(), m_nsyn * 4);
+		dPl.upload(Pl.data(), m_nsyn);
+		dDl.upload(Dl.data(), m_nsyn);
+		dDense.upload(m_dense.data(), m_dense.size() * 4);
+		dLsrc.alloc(m_nsyn * 4);
+		dLtgt.alloc(m_nsyn * 4);
+		dWfx.alloc(m_nsyn * 4);
+		dBound.alloc(size_t(2) * m_n * 8);
+		dFlags.alloc(4);
+		hipCheck(nemo_dev_fill_u64(dBound.as<uint64_t>(), 0, size_t(2) * m_n, bs), "fill");
+		hipCheck(nemo_dev_fill_u32(dFlags.as<uint32_t>(), 0, 1, bs), "fill");
+		hipCheck(nemo_dev_build_map(dS.as<uint32_t>(), dTg.as<uint32_t>(), dW.as<float>(), dPl.as<uint8_t>(),
+					m_nsyn, dDense.as<uint32_t>(), m_minG, m_dense.size(), m_stdp ? 1 : 0, m_sp.maxE, m_sp.maxI,
+					dLsrc.as<uint32_t>(), dLtgt.as<uint32_t>(), dWfx.as<int32_t>(), dBound.as<int64_t>(),
+					dBound.as<int64_t>() + m_n, m_n, dFlags.as<uint32_t>(), bs), "construction kernel");
+		hipCheck(hipMemcpyAsync(&bflags, dFlags.ptr, 4, hipMemcpyDeviceToHost, bs), "hipMemcpyAsync D2H");
+		hipCheck(hipStreamSynchronize(bs), "hipStreamSynchronize");
+		buildLock.unlock();
+		if(bflags & NEMO_BUILD_HOST_CHECK) {
+			onDevice = false;
 		}
-		uint32_t lt = (t >= m_minG && size_t(t - m_minG) < m_dense.size()) ? m_dense[t - m_minG] : 0u;
-		if(!lt) {
-			throw exception(NEMO_INVALID_INPUT, fmt("Invalid synapse target neuron ", t, " (source: ", s, ")"));
-		}
-		lsrc[i] = ls - 1;
-		ltgt[i] = lt - 1;
-		int32_t w = fx_toFix(W[i]);   /* src/nemo/ConnectivityMatrix.cpp:101 */
-		wfx[i] = w;
-		int64_t wb = w;
-		if(m_stdp && Pl[i]) {
-			if(w > 0) wb = std::max<int64_t>(w, m_sp.maxE);
-			else if(w < 0) wb = std::min<int64_t>(w, m_sp.maxI);
-		}
-		if(w >= 0) boundE[lt - 1] += wb; else boundI[lt - 1] += wb;
-		if(m_stdp && Pl[i]) m_nplastic += 1;
 	}
-	for(uint32_t l = 0; l < m_n; ++l) {
-		if(boundE[l] > int64_t(INT32_MAX) || boundI[l] < int64_t(INT32_MIN)) {
-			m_saturate = true;
-			break;
+	std::vector<uint32_t> lsrc, ltgt;
+	std::vector<int32_t> wfx;
+	if(onDevice) {
+		m_saturate = (bflags & NEMO_BUILD_SATURATE) != 0;
+		if(m_stdp) {
+			m_nplastic = size_t(std::count_if(Pl.begin(), Pl.end(), [](unsigned char c) { return c != 0; }));
+		}
+	} else {
+		lsrc.resize(m_nsyn);
+		ltgt.resize(m_nsyn);
+		wfx.resize(m_nsyn);
+		std::vector<int64_t> boundE(m_n, 0), boundI(m_n, 0);
+		for(size_t i = 0; i < m_nsyn; ++i) {
+			unsigned s = S[i], t = Tg[i];
+			uint32_t ls = (s >= m_minG && size_t(s - m_minG) < m_dense.size()) ? m_dense[s - m_minG] : 0u;
+			if(!ls) {
+				throw exception(NEMO_INVALID_INPUT, fmt("Invalid synapse source neuron ", s));
+			}
+			uint32_t lt = (t >= m_minG && size_t(t - m_minG) < m_dense.size()) ? m_dense[t - m_minG] : 0u;
+			if(!lt) {
+				throw exception(NEMO_INVALID_INPUT, fmt("Invalid synapse target neuron ", t, " (source: ", s, ")"));
+			}
+			lsrc[i] = ls - 1;
+			ltgt[i] = lt - 1;
+			int32_t w = fx_toFix(W[i]);   /* src/nemo/ConnectivityMatrix.cpp:101 */
+			wfx[i] = w;
+			int64_t wb = w;
+			if(m_stdp && Pl[i]) {
+				if(w > 0) wb = std::max<int64_t>(w, m_sp.maxE);
+				else if(w < 0) wb = std::min<int64_t>(w, m_sp.maxI);
+			}
+			if(w >= 0) boundE[lt - 1] += wb; else boundI[lt - 1] += wb;
+			if(m_stdp && Pl[i]) m_nplastic += 1;
+		}
+		for(uint32_t l = 0; l < m_n; ++l) {
+			if(boundE[l] > int64_t(INT32_MAX) || boundI[l] < int64_t(INT32_MIN)) {
+				m_saturate = true;
+				break;
+			}
 		}
 	}
 
@@ -331,110 +377,130 @@ Simulation::Simulation(const NetworkImpl& net, const ConfigurationImpl& conf) :
 		}
 	}
 
-	/* ---- forward matrix, bucketed by (source, target partition) */
+	if(onDevice && !m_fused) {
+		/* the plugin step's layout is built on the host, from the mapped synapses */
+		lsrc.resize(m_nsyn);
+		ltgt.resize(m_nsyn);
+		wfx.resize(m_nsyn);
+		buildLock.lock();
+		hipCheck(hipMemcpy(lsrc.data(), dLsrc.ptr, m_nsyn * 4, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+		hipCheck(hipMemcpy(ltgt.data(), dLtgt.ptr, m_nsyn * 4, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+		hipCheck(hipMemcpy(wfx.data(), dWfx.ptr, m_nsyn * 4, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+		buildLock.unlock();
+		onDevice = false;
+	}
+	m_deviceBuilt = onDevice;
 	const uint32_t P1 = m_P + 1;
 	const uint32_t NEAR_SHIFT = 28, NEAR_MAX = 15;
-	std::vector<uint32_t> nearCount;
-	std::vector<uint32_t> off(size_t(m_n) * P1 + 1, 0u);
-	for(size_t i = 0; i < m_nsyn; ++i) {
-		off[size_t(lsrc[i]) * P1 + (m_fused ? ltgt[i] / T : 0u)] += 1;
-	}
-	uint64_t run = 0;
-	for(size_t k = 0; k < off.size(); ++k) {
-		uint64_t c = off[k];
-		off[k] = uint32_t(run);
-		run += c;
-	}
-	if(run >= (uint64_t(1) << 32)) {
-		throw exception(NEMO_API_UNSUPPORTED, "more than 2^32 synapses in one simulation");
-	}
-	std::vector<uint32_t> cursor(off.begin(), off.end());
-	std::vector<uint32_t> syn(2 * m_nsyn);
-	std::vector<uint32_t> fwd(m_nsyn);
-	/* fused step: the delay-1 records of each bucket come first (pass 0),
-	 * so that the kernel can deliver them in the cycle and the rest one
-	 * cycle later */
-	for(int pass = 0; pass < (m_fused ? 2 : 1); ++pass) {
+	std::vector<uint32_t> off, syn;
+	std::vector<uint32_t> rcmStart, rcmSd, rcmFwd;
+	if(onDevice) {
+		buildForwardOnDevice(Tg, Dl, Pl, bflags, dLsrc.as<uint32_t>(), dLtgt.as<uint32_t>(), dWfx.as<int32_t>(),
+				dDl.as<uint8_t>(), dPl.as<uint8_t>());
+	} else {
+		/* ---- forward matrix, bucketed by (source, target partition) */
+		std::vector<uint32_t> nearCount;
+		off.assign(size_t(m_n) * P1 + 1, 0u);
 		for(size_t i = 0; i < m_nsyn; ++i) {
-			if(m_fused && ((Dl[i] == 1) != (pass == 0))) continue;
-			uint32_t lt = ltgt[i];
-			uint32_t p = m_fused ? lt / T : 0u;
-			uint32_t pos = cursor[size_t(lsrc[i]) * P1 + p]++;
-			if(m_fused) {
-				syn[2 * size_t(pos)] = (lt - p * T) | (uint32_t(Dl[i]) << 16);
-			} else {
-				syn[2 * size_t(pos)] = lt | (uint32_t(Dl[i] - 1) << 26);
-			}
-			syn[2 * size_t(pos) + 1] = uint32_t(wfx[i]);
-			fwd[i] = pos;
+			off[size_t(lsrc[i]) * P1 + (m_fused ? ltgt[i] / T : 0u)] += 1;
 		}
-		if(m_fused && pass == 0) {
-			/* cursor - off = delay-1 records per bucket; the deferred form
-			 * keeps that count in the top bits of the bucket start */
-			m_defer = run < (uint64_t(1) << NEAR_SHIFT);
-			for(size_t k = 0; k + 1 < off.size() && m_defer; ++k) {
-				if(cursor[k] - off[k] > NEAR_MAX) m_defer = false;
+		uint64_t run = 0;
+		for(size_t k = 0; k < off.size(); ++k) {
+			uint64_t c = off[k];
+			off[k] = uint32_t(run);
+			run += c;
+		}
+		if(run >= (uint64_t(1) << 32)) {
+			throw exception(NEMO_API_UNSUPPORTED, "more than 2^32 synapses in one simulation");
+		}
+		std::vector<uint32_t> cursor(off.begin(), off.end());
+		syn.assign(2 * m_nsyn, 0u);
+		std::vector<uint32_t> fwd(m_nsyn);
+		/* fused step: the delay-1 records of each bucket come first (pass 0),
+		 * so that the kernel can deliver them in the cycle and the rest one
+		 * cycle later */
+		for(int pass = 0; pass < (m_fused ? 2 : 1); ++pass) {
+			for(size_t i = 0; i < m_nsyn; ++i) {
+				if(m_fused && ((Dl[i] == 1) != (pass == 0))) continue;
+				uint32_t lt = ltgt[i];
+				uint32_t p = m_fused ? lt / T : 0u;
+				uint32_t pos = cursor[size_t(lsrc[i]) * P1 + p]++;
+				if(m_fused) {
+					syn[2 * size_t(pos)] = (lt - p * T) | (uint32_t(Dl[i]) << 16);
+				} else {
+					syn[2 * size_t(pos)] = lt | (uint32_t(Dl[i] - 1) << 26);
+				}
+				syn[2 * size_t(pos) + 1] = uint32_t(wfx[i]);
+				fwd[i] = pos;
 			}
-			if(m_defer) {
-				for(size_t k = 0; k + 1 < off.size(); ++k) {
-					nearCount.push_back(cursor[k] - off[k]);
+			if(m_fused && pass == 0) {
+				/* cursor - off = delay-1 records per bucket; the deferred form
+				 * keeps that count in the top bits of the bucket start */
+				m_defer = run < (uint64_t(1) << NEAR_SHIFT);
+				for(size_t k = 0; k + 1 < off.size() && m_defer; ++k) {
+					if(cursor[k] - off[k] > NEAR_MAX) m_defer = false;
+				}
+				if(m_defer) {
+					for(size_t k = 0; k + 1 < off.size(); ++k) {
+						nearCount.push_back(cursor[k] - off[k]);
+					}
 				}
 			}
 		}
-	}
-	if(m_defer) {
-		for(size_t k = 0; k < nearCount.size(); ++k) {
-			off[k] |= nearCount[k] << NEAR_SHIFT;
+		if(m_defer) {
+			for(size_t k = 0; k < nearCount.size(); ++k) {
+				off[k] |= nearCount[k] << NEAR_SHIFT;
+			}
+			std::vector<uint32_t>().swap(nearCount);
 		}
-		std::vector<uint32_t>().swap(nearCount);
-	}
-	if(!m_fused) {
-		/* rows per source: off[s] .. off[s+1] */
-		std::vector<uint32_t> rows(size_t(m_n) + 1, 0u);
-		for(uint32_t s = 0; s <= m_n; ++s) rows[s] = off[size_t(s) * 2];
-		off.swap(rows);
-	}
-	std::vector<uint32_t>().swap(cursor);
-	std::vector<int32_t>().swap(wfx);
+		if(!m_fused) {
+			/* rows per source: off[s] .. off[s+1] */
+			std::vector<uint32_t> rows(size_t(m_n) + 1, 0u);
+			for(uint32_t s = 0; s <= m_n; ++s) rows[s] = off[size_t(s) * 2];
+			off.swap(rows);
+		}
+		std::vector<uint32_t>().swap(cursor);
+		std::vector<int32_t>().swap(wfx);
 
-	/* ---- synapse query tables, ordinal order per source */
-	if(!m_writeOnly) {
-		m_qStart.assign(size_t(m_n) + 1, 0);
-		for(size_t i = 0; i < m_nsyn; ++i) m_qStart[lsrc[i] + 1] += 1;
-		for(uint32_t l = 0; l < m_n; ++l) m_qStart[l + 1] += m_qStart[l];
-		std::vector<uint64_t> qc(m_qStart.begin(), m_qStart.end() - 1);
-		m_qTarget.resize(m_nsyn);
-		m_qDelay.resize(m_nsyn);
-		m_qPlastic.resize(m_nsyn);
-		m_qFwd.resize(m_nsyn);
-		for(size_t i = 0; i < m_nsyn; ++i) {
-			uint64_t q = qc[lsrc[i]]++;
-			m_qTarget[q] = Tg[i];
-			m_qDelay[q] = Dl[i];
-			m_qPlastic[q] = Pl[i];
-			m_qFwd[q] = fwd[i];
+		/* ---- synapse query tables, ordinal order per source */
+		if(!m_writeOnly) {
+			m_qStart.assign(size_t(m_n) + 1, 0);
+			for(size_t i = 0; i < m_nsyn; ++i) m_qStart[lsrc[i] + 1] += 1;
+			for(uint32_t l = 0; l < m_n; ++l) m_qStart[l + 1] += m_qStart[l];
+			std::vector<uint64_t> qc(m_qStart.begin(), m_qStart.end() - 1);
+			m_qTarget.resize(m_nsyn);
+			m_qDelay.resize(m_nsyn);
+			m_qPlastic.resize(m_nsyn);
+			m_qFwd.resize(m_nsyn);
+			for(size_t i = 0; i < m_nsyn; ++i) {
+				uint64_t q = qc[lsrc[i]]++;
+				m_qTarget[q] = Tg[i];
+				m_qDelay[q] = Dl[i];
+				m_qPlastic[q] = Pl[i];
+				m_qFwd[q] = fwd[i];
+			}
 		}
-	}
 
-	/* ---- reverse matrix of plastic synapses (src/nemo/construction/RCM.cpp:150-171) */
-	std::vector<uint32_t> rcmStart, rcmSd, rcmFwd;
-	if(m_stdp) {
-		rcmStart.assign(size_t(m_n) + 1, 0);
-		for(size_t i = 0; i < m_nsyn; ++i) if(Pl[i]) rcmStart[ltgt[i] + 1] += 1;
-		for(uint32_t l = 0; l < m_n; ++l) rcmStart[l + 1] += rcmStart[l];
-		std::vector<uint32_t> rc(rcmStart.begin(), rcmStart.end() - 1);
-		rcmSd.resize(m_nplastic);
-		rcmFwd.resize(m_nplastic);
-		for(size_t i = 0; i < m_nsyn; ++i) {
-			if(!Pl[i]) continue;
-			uint32_t r = rc[ltgt[i]]++;
-			rcmSd[r] = lsrc[i] | (uint32_t(Dl[i] - 1) << 26);
-			rcmFwd[r] = fwd[i];
+		/* ---- reverse matrix of plastic synapses (src/nemo/construction/RCM.cpp:150-171) */
+		if(m_stdp) {
+			rcmStart.assign(size_t(m_n) + 1, 0);
+			for(size_t i = 0; i < m_nsyn; ++i) if(Pl[i]) rcmStart[ltgt[i] + 1] += 1;
+			for(uint32_t l = 0; l < m_n; ++l) rcmStart[l + 1] += rcmStart[l];
+			std::vector<uint32_t> rc(rcmStart.begin(), rcmStart.end() - 1);
+			rcmSd.resize(m_nplastic);
+			rcmFwd.resize(m_nplastic);
+			for(size_t i = 0; i < m_nsyn; ++i) {
+				if(!Pl[i]) continue;
+				uint32_t r = rc[ltgt[i]]++;
+				rcmSd[r] = lsrc[i] | (uint32_t(Dl[i] - 1) << 26);
+				rcmFwd[r] = fwd[i];
+			}
 		}
+		std::vector<uint32_t>().swap(lsrc);
+		std::vector<uint32_t>().swap(ltgt);
+		std::vector<uint32_t>().swap(fwd);
 	}
-	std::vector<uint32_t>().swap(lsrc);
-	std::vector<uint32_t>().swap(ltgt);
-	std::vector<uint32_t>().swap(fwd);
+	dS.alloc(0); dTg.alloc(0); dW.alloc(0); dPl.alloc(0); dDl.alloc(0); dLsrc.alloc(0); dLtgt.alloc(0); dWfx.alloc(0);
 
 	/* ---- neuron arrays */
 	std::vector<float> param(size_t(m_np) * m_n, 0.0f);
@@ -466,12 +532,14 @@ Simulation::Simulation(const NetworkImpl& net, const ConfigurationImpl& conf) :
 	d_rng.upload(rng.data(), rng.size() * sizeof(uint32_t));
 	d_recent.alloc(size_t(m_n) * 8);
 	d_ring.alloc(size_t(m_P) * 2 * m_D * m_T * 4);
-	d_off.upload(off.data(), off.size() * sizeof(uint32_t));
-	if(syn.size() < 2) {
-		/* the delivery loop reads record 0 in place of an absent record */
-		syn.resize(2, 0u);
+	if(!m_deviceBuilt) {
+		d_off.upload(off.data(), off.size() * sizeof(uint32_t));
+		if(syn.size() < 2) {
+			/* the delivery loop reads record 0 in place of an absent record */
+			syn.resize(2, 0u);
+		}
+		d_syn.upload(syn.data(), syn.size() * sizeof(uint32_t));
 	}
-	d_syn.upload(syn.data(), syn.size() * sizeof(uint32_t));
 	/* fired lists and counts are kept for flRing cycles: 2 (parity), 3 when
 	 * deferred delivery reads the previous cycle's, postLen + 2 with STDP,
 	 * whose accumulation reads those of cycle t - postLen */
@@ -481,10 +549,12 @@ Simulation::Simulation(const NetworkImpl& net, const ConfigurationImpl& conf) :
 	d_ctl.alloc(64);
 	if(m_stdp) {
 		d_rfg.alloc(size_t(2) * m_n * 8);
-		d_rcmStart.upload(rcmStart.data(), rcmStart.size() * 4);
-		d_rcmSd.upload(rcmSd.data(), rcmSd.size() * 4);
-		d_rcmFwd.upload(rcmFwd.data(), rcmFwd.size() * 4);
-		d_rcmAcc.alloc(std::max<size_t>(rcmSd.size(), 1) * 4);
+		if(!m_deviceBuilt) {
+			d_rcmStart.upload(rcmStart.data(), rcmStart.size() * 4);
+			d_rcmSd.upload(rcmSd.data(), rcmSd.size() * 4);
+			d_rcmFwd.upload(rcmFwd.data(), rcmFwd.size() * 4);
+		}
+		d_rcmAcc.alloc(std::max<size_t>(m_nplastic, 1) * 4);
 		d_stdp.upload(&m_sp, sizeof(m_sp));
 		if(m_fused) {
 			/* reverse row {start, length} of each fired neuron, beside its
@@ -533,6 +603,131 @@ Simulation::Simulation(const NetworkImpl& net, const ConfigurationImpl& conf) :
 				m_P, m_T, m_D, m_saturate ? "saturating" : "int32", double(m_deviceBytes) / 1e6);
 	}
 	resetTimer();
+}
+
+void
+Simulation::buildForwardOnDevice(const std::vector<unsigned>& Tg, const std::vector<unsigned char>& Dl,
+		const std::vector<unsigned char>& Pl, uint32_t bflags, const uint32_t* lsrc, const uint32_t* ltgt,
+		const int32_t* wfx, const uint8_t* dDl, const uint8_t* dPl)
+{
+	std::lock_guard<std::recursive_mutex> lock(deviceMutex());
+	hipStream_t st = static_cast<hipStream_t>(m_stream);
+	const uint32_t P1 = m_P + 1;
+	const size_t nb = size_t(m_n) * P1;
+	const uint32_t NEAR_SHIFT = 28, NEAR_MAX = 15;
+	auto scan = [&](const uint32_t* in, uint32_t* out, size_t n) {
+		size_t tb = 0;
+		hipCheck(nemo_dev_exclusive_scan_u32(in, out, n, nullptr, &tb, st), "scan (size)");
+		DevBuf temp;
+		temp.alloc(std::max<size_t>(tb, 16));
+		hipCheck(nemo_dev_exclusive_scan_u32(in, out, n, temp.ptr, &tb, st), "scan");
+		hipCheck(hipStreamSynchronize(st), "hipStreamSynchronize");
+	};
+
+	/* bucket sizes and their delay-1 part, then bucket starts */
+	DevBuf dCount, dNear, dCursor, dFwd;
+	dCount.alloc((nb + 1) * 4);
+	dNear.alloc(nb * 4);
+	hipCheck(nemo_dev_fill_u32(dCount.as<uint32_t>(), 0, nb + 1, st), "fill");
+	hipCheck(nemo_dev_fill_u32(dNear.as<uint32_t>(), 0, nb, st), "fill");
+	hipCheck(nemo_dev_build_count(lsrc, ltgt, dDl, m_nsyn, m_T, P1, dCount.as<uint32_t>(), dNear.as<uint32_t>(), st),
+			"construction kernel");
+	d_off.alloc((nb + 1) * 4);
+	scan(dCount.as<uint32_t>(), d_off.as<uint32_t>(), nb + 1);
+	dCount.alloc(0);
+
+	/* records: the delay-1 records of every bucket first */
+	dCursor.alloc(nb * 4);
+	hipCheck(hipMemcpyAsync(dCursor.ptr, d_off.ptr, nb * 4, hipMemcpyDeviceToDevice, st), "hipMemcpyAsync D2D");
+	d_syn.alloc(std::max<size_t>(m_nsyn, 1) * 8);
+	dFwd.alloc(m_nsyn * 4);
+	for(int pass = 0; pass < 2; ++pass) {
+		hipCheck(nemo_dev_build_scatter(lsrc, ltgt, dDl, wfx, m_nsyn, m_T, P1, pass, dCursor.as<uint32_t>(),
+					d_syn.as<uint32_t>(), dFwd.as<uint32_t>(), st), "construction kernel");
+	}
+	dCursor.alloc(0);
+
+	/* per-source query starts (before the delay-1 counts go into off) */
+	if(!m_writeOnly && !(bflags & NEMO_BUILD_UNORDERED)) {
+		DevBuf dq;
+		dq.alloc((size_t(m_n) + 1) * 4);
+		hipCheck(nemo_dev_build_gather(d_off.as<uint32_t>(), P1, m_n, dq.as<uint32_t>(), st), "construction kernel");
+		std::vector<uint32_t> q(size_t(m_n) + 1);
+		hipCheck(hipMemcpyAsync(q.data(), dq.ptr, q.size() * 4, hipMemcpyDeviceToHost, st), "hipMemcpyAsync D2H");
+		hipCheck(hipStreamSynchronize(st), "hipStreamSynchronize");
+		m_qStart.assign(q.begin(), q.end());
+	}
+
+	/* deferred delivery: every bucket holds at most NEAR_MAX delay-1 records
+	 * and the record indices leave the top bits free */
+	m_defer = uint64_t(m_nsyn) < (uint64_t(1) << NEAR_SHIFT);
+	if(m_defer) {
+		DevBuf dMax;
+		dMax.alloc(4);
+		uint32_t mx = 0;
+		hipCheck(nemo_dev_fill_u32(dMax.as<uint32_t>(), 0, 1, st), "fill");
+		hipCheck(nemo_dev_build_max(dNear.as<uint32_t>(), nb, dMax.as<uint32_t>(), st), "construction kernel");
+		hipCheck(hipMemcpyAsync(&mx, dMax.ptr, 4, hipMemcpyDeviceToHost, st), "hipMemcpyAsync D2H");
+		hipCheck(hipStreamSynchronize(st), "hipStreamSynchronize");
+		m_defer = mx <= NEAR_MAX;
+	}
+	if(m_defer) {
+		hipCheck(nemo_dev_build_near_pack(d_off.as<uint32_t>(), dNear.as<uint32_t>(), nb, NEAR_SHIFT, st),
+				"construction kernel");
+	}
+	dNear.alloc(0);
+
+	/* synapse query tables, ordinal order per source: with the sources in
+	 * ascending local order that is the network's own order */
+	if(!m_writeOnly) {
+		std::vector<uint32_t> fwd(m_nsyn);
+		hipCheck(hipMemcpyAsync(fwd.data(), dFwd.ptr, m_nsyn * 4, hipMemcpyDeviceToHost, st), "hipMemcpyAsync D2H");
+		std::vector<uint32_t> ls;
+		if(bflags & NEMO_BUILD_UNORDERED) {
+			ls.resize(m_nsyn);
+			hipCheck(hipMemcpyAsync(ls.data(), lsrc, m_nsyn * 4, hipMemcpyDeviceToHost, st), "hipMemcpyAsync D2H");
+		}
+		hipCheck(hipStreamSynchronize(st), "hipStreamSynchronize");
+		if(bflags & NEMO_BUILD_UNORDERED) {
+			m_qStart.assign(size_t(m_n) + 1, 0);
+			for(size_t i = 0; i < m_nsyn; ++i) m_qStart[ls[i] + 1] += 1;
+			for(uint32_t l = 0; l < m_n; ++l) m_qStart[l + 1] += m_qStart[l];
+			std::vector<uint64_t> qc(m_qStart.begin(), m_qStart.end() - 1);
+			m_qTarget.resize(m_nsyn);
+			m_qDelay.resize(m_nsyn);
+			m_qPlastic.resize(m_nsyn);
+			m_qFwd.resize(m_nsyn);
+			for(size_t i = 0; i < m_nsyn; ++i) {
+				uint64_t q = qc[ls[i]]++;
+				m_qTarget[q] = Tg[i];
+				m_qDelay[q] = Dl[i];
+				m_qPlastic[q] = Pl[i];
+				m_qFwd[q] = fwd[i];
+			}
+		} else {
+			m_qTarget.assign(Tg.begin(), Tg.end());
+			m_qDelay.assign(Dl.begin(), Dl.end());
+			m_qPlastic.assign(Pl.begin(), Pl.end());
+			m_qFwd.swap(fwd);
+		}
+	}
+
+	/* reverse matrix of plastic synapses (src/nemo/construction/RCM.cpp:150-171) */
+	if(m_stdp) {
+		DevBuf dRc;
+		dRc.alloc((size_t(m_n) + 1) * 4);
+		hipCheck(nemo_dev_fill_u32(dRc.as<uint32_t>(), 0, size_t(m_n) + 1, st), "fill");
+		hipCheck(nemo_dev_build_rcm_count(ltgt, dPl, m_nsyn, dRc.as<uint32_t>(), st), "construction kernel");
+		d_rcmStart.alloc((size_t(m_n) + 1) * 4);
+		scan(dRc.as<uint32_t>(), d_rcmStart.as<uint32_t>(), size_t(m_n) + 1);
+		hipCheck(hipMemcpyAsync(dRc.ptr, d_rcmStart.ptr, size_t(m_n) * 4, hipMemcpyDeviceToDevice, st),
+				"hipMemcpyAsync D2D");
+		d_rcmSd.alloc(std::max<size_t>(m_nplastic, 1) * 4);
+		d_rcmFwd.alloc(std::max<size_t>(m_nplastic, 1) * 4);
+		hipCheck(nemo_dev_build_rcm_scatter(lsrc, ltgt, dDl, dPl, dFwd.as<uint32_t>(), m_nsyn, dRc.as<uint32_t>(),
+					d_rcmSd.as<uint32_t>(), d_rcmFwd.as<uint32_t>(), st), "construction kernel");
+	}
+	hipCheck(hipStreamSynchronize(st), "hipStreamSynchronize");
 }
 
 /* Load lib<Model>.so for every neuron type (src/nemo/cuda/Neurons.cpp:43-44,
@@ -972,6 +1167,7 @@ Simulation::info(nemo_hip_info_t* out) const
 	out->lanes_per_source = m_fused ? (1u << m_glog2) : 0u;
 	out->stdp_staged_chunks = m_stdpCache;
 	out->stdp_prefix_ring = m_prefRing ? 1u : 0u;
+	out->device_built = m_deviceBuilt ? 1u : 0u;
 }
 
 /* src/nemo/ConnectivityMatrix.cpp:241-294 */

@@ -11,6 +11,7 @@
 #include "../device/models.h"
 #include "../device/step_abi.h"
 #include "../device/plugin_abi.h"
+#include "../device/build_abi.h"
 #include "nemo_hip.h"
 #include "nemo_hip_plugin.h"
 
@@ -29,6 +30,7 @@ struct ConfigurationImpl {
 	unsigned partitionSize = 0;   /* 0 = automatic */
 	unsigned deliveryLanes = 0;   /* lanes per fired source in delivery; 0 = automatic */
 	bool stdpLdsStaging = true;   /* STDP count prefixes and reverse records staged in LDS */
+	bool deviceConstruction = true;   /* synapse structures built on the device */
 	int stepMode = NEMO_STEP_AUTO;
 	std::string description;
 
@@ -197,6 +199,13 @@ class Simulation
 		void writeScalar(DevBuf& b, size_t offset, const void* in, size_t bytes);
 		void decodeStep(const uint64_t* words, std::vector<unsigned>& out) const;
 		size_t synapseRecord(synapse_id id) const;
+		/* the forward matrix (d_off, d_syn, m_defer), the synapse query
+		 * tables and, with STDP, the reverse matrix, built on the device from
+		 * the mapped synapses (device arrays of m_nsyn entries) */
+		void buildForwardOnDevice(const std::vector<unsigned>& Tg, const std::vector<unsigned char>& Dl,
+				const std::vector<unsigned char>& Pl, uint32_t bflags, const uint32_t* lsrc, const uint32_t* ltgt,
+				const int32_t* wfx, const uint8_t* dDl, const uint8_t* dPl);
+		bool m_deviceBuilt = false;   /* synapse structures built on the device */
 };
 
 } // namespace hip

@@ -100,6 +100,10 @@ class Configuration(object):
         """STDP count prefixes and reverse records staged in LDS (default on; same results)."""
         _check(self._lib.nemo_set_stdp_lds_staging(self._h, 1 if enable else 0))
 
+    def set_device_construction(self, enable):
+        """Synapse structures built on the device (default) or on the host; same results."""
+        _check(self._lib.nemo_set_device_construction(self._h, 1 if enable else 0))
+
     def set_step_mode(self, mode):
         """'auto', 'fused' or 'plugins' (see nemo_set_step_mode)."""
         _check(self._lib.nemo_set_step_mode(self._h, STEP_MODES[mode]))

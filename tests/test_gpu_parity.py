@@ -230,6 +230,115 @@ def test_stdp_work_lists(nemo_mod, oracle_mod, staging):
     _compare_weights(sim, orc, spec, 3000)
 
 
+def _construction_pair(nemo_mod, fill, stdp=None, step_mode=None):
+    """The same network built with device and with host construction."""
+    from backends import HipBuilder
+    sims = []
+    for dev in (True, False):
+        b = HipBuilder(nemo_mod)
+        fill(b)
+        if stdp is not None:
+            b.set_stdp(*stdp)
+        b.conf.set_device_construction(dev)
+        sims.append(b.simulation(step_mode=step_mode))
+    return sims
+
+
+def _same_simulation(sims, n, cycles, sources, nsyn_check=True):
+    dev, host = sims
+    idev, ihost = dev.info(), host.info()
+    for k in ("partitions", "partition_size", "saturating", "deferred", "fused", "synapses", "plastic_synapses"):
+        assert idev[k] == ihost[k], (k, idev, ihost)
+    dev.run(cycles)
+    host.run(cycles)
+    dc, di = dev.read_fired()
+    hc, hi = host.read_fired()
+    d = first_divergence(hc, hi, dc, di)
+    assert d is None, "first diverging cycle %d: %s" % d
+    for src in sources:
+        ids = dev.get_synapses_from(src)
+        assert list(ids) == list(host.get_synapses_from(src))
+        for sid in ids:
+            assert dev.get_synapse_target(sid) == host.get_synapse_target(sid)
+            assert dev.get_synapse_delay(sid) == host.get_synapse_delay(sid)
+            assert dev.get_synapse_weight(sid) == host.get_synapse_weight(sid)
+            assert dev.get_synapse_plastic(sid) == host.get_synapse_plastic(sid)
+    return idev
+
+
+@pytest.mark.parametrize("kind", ["csr", "stdp", "shuffled", "dense_near", "saturating", "plugin"])
+def test_device_construction(nemo_mod, kind):
+    """Synapse structures built on the device (atomic bucket cursors, so the
+    records of a bucket land in any order) and on the host give the same
+    simulation: geometry and mode decisions, rasters, synapse queries and,
+    with STDP, weights after applyStdp. Covers networks given as CSR (sources
+    in order), synapse by synapse in shuffled order, with buckets of more
+    than 15 delay-1 records (no deferred delivery), with saturating sums,
+    and on the plugin step (built on the host from the device's mapping)."""
+    rng = np.random.default_rng(3)
+    stdp = None
+    step_mode = None
+    if kind in ("csr", "stdp", "dense_near", "plugin"):
+        spec = examples.izhikevich_80_20(3000 if kind != "stdp" else 2000, 150, seed=31, stdp=(kind == "stdp"))
+
+        def fill(b):
+            b.load_spec(spec)
+            if kind == "dense_near":
+                for s in range(20):
+                    for t in range(40):
+                        b.add_synapse(s, t, 1, 0.3, 0)
+        n = spec.n
+        if kind == "stdp":
+            pre, post = examples.stdp_window()
+            stdp = (pre, post, 0.0, 1.0, 0.0, -1.0)
+        if kind == "plugin":
+            step_mode = "plugins"
+    else:
+        spec = examples.izhikevich_80_20(1000, 40, seed=32)
+        order = rng.permutation(spec.nsyn)
+        src_of = np.searchsorted(spec.rowptr, np.arange(spec.nsyn), side="right") - 1
+        scale = 3000.0 if kind == "saturating" else 1.0
+
+        def fill(b):
+            for name, idx, args in spec.groups:
+                if name not in b.types:
+                    b.types[name] = b.net.add_neuron_type(name)
+                b.net.add_neurons(b.types[name], idx, args)
+            for k in order:
+                w = float(spec.weights[k]) * scale
+                w = max(-2047.0, min(2047.0, w))
+                b.add_synapse(int(src_of[k]), int(spec.targets[k]), int(spec.delays[k]), w, 0)
+        n = spec.n
+    sims = _construction_pair(nemo_mod, fill, stdp=stdp, step_mode=step_mode)
+    info = _same_simulation(sims, n, 300, [0, 1, n // 2, n - 1])
+    assert sims[0].info()["device_built"] == (0 if kind == "plugin" else 1)
+    assert sims[1].info()["device_built"] == 0
+    if kind == "dense_near":
+        assert info["deferred"] == 0
+    if kind == "saturating":
+        assert info["saturating"] == 1
+    if kind == "stdp":
+        for sim in sims:
+            sim.apply_stdp(1.0)
+        _same_simulation(sims, n, 100, [0, 5, 999, 1500])
+
+
+def test_device_construction_invalid(nemo_mod):
+    """A synapse to a neuron that does not exist: device construction hands
+    the network to the host's checks, which raise NeMo's error."""
+    from backends import HipBuilder
+    msgs = []
+    for dev in (True, False):
+        b = HipBuilder(nemo_mod)
+        b.load_spec(examples.izhikevich_80_20(500, 20, seed=4))
+        b.add_synapse(3, 777, 2, 0.5, 0)
+        b.conf.set_device_construction(dev)
+        with pytest.raises(nemo_mod.NemoError) as e:
+            b.simulation()
+        msgs.append(str(e.value))
+    assert msgs[0] == msgs[1] and "777" in msgs[0], msgs
+
+
 def test_synapse_queries(nemo_mod, oracle_mod):
     spec = examples.izhikevich_80_20(300, 20, seed=2)
     sim = _hip_sim(nemo_mod, spec)
