@@ -3,7 +3,8 @@
 Requests (a network to build and a number of 1 ms cycles to run, optionally
 with STDP and periodic applyStdp) arrive over time. The server
 
-  * admits a request when its estimated device footprint fits the HBM budget
+  * admits a request when its estimated device footprint, plus the memory its
+    construction holds on the device while it is built, fits the HBM budget
     (requests wait in arrival order otherwise);
   * builds admitted networks on host worker threads while the GPU runs;
   * keeps every admitted simulation resident on the device and advances all
@@ -69,6 +70,16 @@ class Request(object):
         if self.stdp:
             b += 12 * int(0.8 * n * m) + 16 * n
         return int(b * 1.1) + (1 << 20)
+
+    def construction_bytes(self):
+        """Device memory held only while the network is built on the device:
+        the uploaded synapses and their mapping (30 B/synapse: source,
+        target, weight, local source, local target, Q11.20 weight, delay,
+        plastic flag, record index) and three bucket counters per (source,
+        partition)."""
+        n, m = self.neurons, self.synapses
+        p = max(1, (n + 255) // 256)
+        return 30 * n * m + 12 * n * (p + 1) + (1 << 20)
 
 
 def build_simulation(req):
@@ -157,7 +168,7 @@ class Server(object):
             # admission by budget, in arrival order
             while waiting:
                 r = waiting[0]
-                need = r.estimated_device_bytes()
+                need = r.estimated_device_bytes() + r.construction_bytes()
                 if need > self.budget:
                     raise ValueError("request %d needs %d B, more than the whole budget" % (r.rid, need))
                 if self.used + need > self.budget:
@@ -171,6 +182,7 @@ class Server(object):
             for fut, r in building:
                 if fut.done():
                     r.sim = fut.result()
+                    self.used -= r.construction_bytes()
                     active.append(r)
                 else:
                     still.append((fut, r))
