@@ -17,7 +17,8 @@
 namespace nemo_dev {
 
 /* Neuron model ids (the order of nemosim's built-in model table) */
-enum { MODEL_IZHIKEVICH = 0, MODEL_IF_CURR_EXP = 1, MODEL_IF_CURR_EXP_NOISE = 2 };
+enum { MODEL_IZHIKEVICH = 0, MODEL_IF_CURR_EXP = 1, MODEL_IF_CURR_EXP_NOISE = 2,
+	MODEL_POISSON_SOURCE = 3, MODEL_INPUT = 4 };
 
 /* Largest parameter / state counts over the built-in models; device arrays
  * are [MAX_PARAM][n] and [MAX_STATE][n]. */
@@ -42,6 +43,8 @@ NEMO_HD inline ModelInfoDev model_info_dev(int m)
 	ModelInfoDev r;
 	if(m == MODEL_IZHIKEVICH) { r.nparam = 5; r.nstate = 2; r.normalRng = true; }
 	else if(m == MODEL_IF_CURR_EXP) { r.nparam = 9; r.nstate = 4; r.normalRng = false; }
+	else if(m == MODEL_POISSON_SOURCE) { r.nparam = 1; r.nstate = 0; r.normalRng = true; }
+	else if(m == MODEL_INPUT) { r.nparam = 0; r.nstate = 0; r.normalRng = false; }
 	else { r.nparam = 10; r.nstate = 4; r.normalRng = true; }
 	return r;
 }
@@ -150,11 +153,31 @@ NEMO_HD inline bool if_curr_exp(const float* p, float* s, Rng& rng, bool with_no
 	return fired;
 }
 
+/* PoissonSource, src/nemo/cpu/plugins/PoissonSource.cpp:22-29: parameter 0
+ * is the firing probability per cycle; fires when the low 16 bits of the
+ * neuron's next urand are below rate * 2^16, or when forced; input is
+ * ignored. The float -> unsigned conversion goes through int64, which is
+ * what the host compiler's conversion does (so a negative or NaN rate gives
+ * the host's threshold, not a clamped one). */
+NEMO_HD inline bool poisson_source(const float* p, Rng& rng, bool fstim)
+{
+	const uint32_t threshold = uint32_t(int64_t(p[0] * 65536.0f));
+	const uint32_t draw = urand(rng) & 0xffffu;
+	return draw < threshold || fstim;
+}
+
 NEMO_HD inline bool update_neuron(int model, const float* p, float* s, Rng& rng,
 		float currentE, float currentI, float currentExt, bool fstim)
 {
 	if(model == MODEL_IZHIKEVICH) {
 		return izhikevich(p, s, rng, currentE, currentI, currentExt, fstim);
+	}
+	if(model == MODEL_POISSON_SOURCE) {
+		return poisson_source(p, rng, fstim);
+	}
+	if(model == MODEL_INPUT) {
+		/* Input, src/nemo/cpu/plugins/Input.cpp:22-27: fires exactly when forced */
+		return fstim;
 	}
 	return if_curr_exp(p, s, rng, model == MODEL_IF_CURR_EXP_NOISE, currentE, currentI, currentExt, fstim);
 }

@@ -39,6 +39,7 @@
  * slice of every row.
  */
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <stdint.h>
 
 #include "models.h"
@@ -225,7 +226,10 @@ constexpr uint32_t OFF_NEAR_SHIFT = 28u;
 /* Delivery rounds whose record loads are issued together (fewer with the
  * saturating accumulation, whose returning atomics need more registers),
  * and records per lane per round (lane i of a group reads records i and i+G) */
-template<bool SAT> struct DeliverRounds { static constexpr uint32_t value = SAT ? 4u : 8u; };
+/* rounds of record loads in flight per batch: fewer where the variant has
+ * fewer registers left (saturating accumulation; any model, 10 parameters
+ * and 4 state variables per neuron) */
+template<bool SAT, bool IZO> struct DeliverRounds { static constexpr uint32_t value = SAT ? 4u : (IZO ? 8u : 6u); };
 constexpr uint32_t DELIVER_KPL = 2u;
 
 /* Partition whose fired list holds flat index j: the q with
@@ -273,7 +277,7 @@ __device__ inline uint32_t pass_rounds(uint32_t nsrc, uint32_t glog)
 	return (nsrc + (64u >> glog) - 1u) >> (6u - glog);
 }
 
-template<bool SAT>
+template<bool SAT, bool IZO>
 __device__ inline void deliver_pass(uint32_t a0, uint32_t b0, uint32_t a1, uint32_t b1, uint32_t nsrc,
 		uint32_t glog, uint32_t lane, uint32_t base, const NEMO_GLOBAL uint2* syn,
 		int32_t* ringE, int32_t* ringI, uint32_t* ovf, uint32_t D, uint32_t T, uint32_t fwords,
@@ -284,7 +288,7 @@ __device__ inline void deliver_pass(uint32_t a0, uint32_t b0, uint32_t a1, uint3
 	const uint32_t grp = lane >> glog;
 	const uint32_t gl = lane & (G - 1u);
 	const uint32_t rounds = rend;
-	constexpr uint32_t DELIVER_RU = DeliverRounds<SAT>::value;
+	constexpr uint32_t DELIVER_RU = DeliverRounds<SAT, IZO>::value;
 	for(uint32_t r0 = rbeg; r0 < rounds; r0 += DELIVER_RU) {    /* wave-uniform */
 		uint2 rec[DELIVER_RU][DELIVER_KPL];
 		uint32_t ok = 0;
@@ -327,7 +331,7 @@ __device__ inline void deliver_pass(uint32_t a0, uint32_t b0, uint32_t a1, uint3
  * among the long ones) and delivered by further passes, so a long bucket
  * costs one more round of loads for the whole wave, not one per bucket.
  * All lanes of the wave must call it together (nsrc is wave-uniform). */
-template<bool SAT>
+template<bool SAT, bool IZO>
 __device__ inline void deliver_ranges(uint32_t a0, uint32_t b0, uint32_t a1, uint32_t b1, uint32_t nsrc,
 		uint32_t glog, uint32_t lane, uint32_t base, const NEMO_GLOBAL uint2* syn,
 		int32_t* ringE, int32_t* ringI, uint32_t* ovf, uint32_t D, uint32_t T, uint32_t fwords)
@@ -336,7 +340,7 @@ __device__ inline void deliver_ranges(uint32_t a0, uint32_t b0, uint32_t a1, uin
 	if(lane >= nsrc) { a0 = b0 = 0u; }
 	if(64u + lane >= nsrc) { a1 = b1 = 0u; }
 	for(;;) {
-		deliver_pass<SAT>(a0, b0, a1, b1, nsrc, glog, lane, base, syn, ringE, ringI, ovf, D, T, fwords,
+		deliver_pass<SAT, IZO>(a0, b0, a1, b1, nsrc, glog, lane, base, syn, ringE, ringI, ovf, D, T, fwords,
 				0u, pass_rounds(nsrc, glog));
 		/* what the pass left */
 		a0 = (b0 - a0 > covered) ? a0 + covered : b0;
@@ -745,7 +749,7 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 		/* ---- deferred records (delays >= 2) of the previous cycle's spikes,
 		 * delivered while the other partitions publish this cycle */
 		if(defer && kn) {
-			deliver_ranges<SAT>(ka0, kb0, ka1, kb1, kn, nd.glog2, lane, (slot + D - 1u) % D, syn,
+			deliver_ranges<SAT, IZO>(ka0, kb0, ka1, kb1, kn, nd.glog2, lane, (slot + D - 1u) % D, syn,
 					ringE, ringI, ovf, D, T, fwords);
 			kn = 0;
 		}
@@ -852,7 +856,7 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 					a0 &= offMask;
 					a1 &= offMask;
 					events += (b0 - a0) + (b1 - a1);
-					deliver_ranges<SAT>(a0, b0, a1, b1, nsrc, nd.glog2, lane, slot, syn,
+					deliver_ranges<SAT, IZO>(a0, b0, a1, b1, nsrc, nd.glog2, lane, slot, syn,
 							ringE, ringI, ovf, D, T, fwords);
 				}
 			}
@@ -944,7 +948,7 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 	/* ---- the last cycle's deferred records */
 	if(defer) {
 		if(kn) {
-			deliver_ranges<SAT>(ka0, kb0, ka1, kb1, kn, nd.glog2, lane,
+			deliver_ranges<SAT, IZO>(ka0, kb0, ka1, kb1, kn, nd.glog2, lane,
 					uint32_t((nd.cycle0 + nd.nsteps - 1u) % D), syn, ringE, ringI, ovf, D, T, fwords);
 		}
 		__syncthreads();
@@ -1185,6 +1189,19 @@ int nemo_dev_step_max_blocks(uint32_t blockDim, uint32_t ldsBytes, uint32_t vari
 	int cus = 0;
 	e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
 	if(e != hipSuccess) return int(e);
+	/* Co-residency measured on gfx950 (workgroups that count themselves in
+	 * and wait, with a time bound, for the rest) falls short of the occupancy
+	 * query when LDS is the limit: each workgroup takes its LDS rounded up to
+	 * 512 bytes plus 512 more (32536 B: 4 per CU, not 5; 16000 B: 9, not 10;
+	 * 8000 B: 18, not 20). A persistent launch needs every workgroup
+	 * resident, so the count follows the measurement. */
+	int ldsPerCU = 0;
+	e = hipDeviceGetAttribute(&ldsPerCU, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev);
+	if(e != hipSuccess) return int(e);
+	if(ldsBytes > 0u && ldsPerCU > 0) {
+		const uint32_t alloc = ((ldsBytes + 511u) & ~511u) + 512u;
+		perCU = std::min(perCU, int(uint32_t(ldsPerCU) / alloc));
+	}
 	*maxBlocks = uint32_t(perCU) * uint32_t(cus);
 	return 0;
 }

@@ -24,6 +24,8 @@ registry()
 		r.push_back(ModelInfo{ "Izhikevich", 0, 5, 2, 1, true, true, "" });
 		r.push_back(ModelInfo{ "IF_curr_exp", 1, 9, 4, 0, false, true, "" });
 		r.push_back(ModelInfo{ "IF_curr_exp_noise", 2, 10, 4, 0, true, true, "" });
+		r.push_back(ModelInfo{ "PoissonSource", 3, 1, 0, 0, true, true, "" });
+		r.push_back(ModelInfo{ "Input", 4, 0, 0, 0, false, true, "" });
 	}
 	return r;
 }

@@ -162,7 +162,9 @@ initialiseRng(unsigned minIdx, unsigned maxIdx, std::vector<RNG>& rngs)
 
 /* ---------------------------------------------------------- neuron types */
 
-enum ModelId { MODEL_IZHIKEVICH = 0, MODEL_IF_CURR_EXP = 1, MODEL_IF_CURR_EXP_NOISE = 2 };
+enum ModelId { MODEL_IZHIKEVICH = 0, MODEL_IF_CURR_EXP = 1, MODEL_IF_CURR_EXP_NOISE = 2,
+	MODEL_POISSON_SOURCE = 3, MODEL_INPUT = 4 };
+static const unsigned NMODELS = 5;
 
 struct TypeInfo {
 	const char* name;
@@ -171,12 +173,15 @@ struct TypeInfo {
 	unsigned membranePotential;
 };
 
-/* src/nemo/plugins/Izhikevich.ini:1-5, src/nemo/plugins/IF_curr_exp.ini:14-26;
+/* src/nemo/plugins/Izhikevich.ini:1-5, src/nemo/plugins/IF_curr_exp.ini:14-26,
+ * src/nemo/plugins/PoissonSource.ini:1-4, src/nemo/plugins/Input.ini:1-4;
  * IF_curr_exp_noise is IF_curr_exp plus a 10th parameter sigma. */
-static const TypeInfo TYPES[3] = {
+static const TypeInfo TYPES[NMODELS] = {
 	{ "Izhikevich", 5, 2, 1 },
 	{ "IF_curr_exp", 9, 4, 0 },
 	{ "IF_curr_exp_noise", 10, 4, 0 },
+	{ "PoissonSource", 1, 0, 0 },
+	{ "Input", 0, 0, 0 },
 };
 
 /* Parameter / state indices: src/nemo/plugins/Izhikevich.h:6-13 */
@@ -243,6 +248,35 @@ update_izhikevich(unsigned start, unsigned end,
 /* src/nemo/cpu/plugins/IF_curr_exp.cpp:27-123; with_noise adds the
  * build-defined Gaussian input (sigma = parameter 9), drawn once per cycle
  * after the total current is formed. */
+/* PoissonSource, src/nemo/cpu/plugins/PoissonSource.cpp:22-29: parameter 0
+ * is the firing probability per cycle; the neuron fires when the low 16 bits
+ * of its next urand are below rate * 2^16, or when forced. Synaptic and
+ * external input are ignored. */
+static void
+update_poisson_source(unsigned start, unsigned end, const float* rate, unsigned fstim[], RNG rng[],
+		uint64_t recentFiring[], unsigned fired[])
+{
+	for(unsigned g = start; g < end; ++g) {
+		const unsigned l = g - start;
+		const unsigned threshold = unsigned(rate[l] * float(1 << 16));
+		const unsigned draw = urand(&rng[l]) & 0xffffu;
+		fired[g] = (draw < threshold || fstim[g]) ? 1u : 0u;
+		fstim[g] = 0;
+		recentFiring[g] = (recentFiring[g] << 1) | uint64_t(fired[g]);
+	}
+}
+
+/* Input, src/nemo/cpu/plugins/Input.cpp:22-27: fires exactly when forced */
+static void
+update_input(unsigned start, unsigned end, unsigned fstim[], uint64_t recentFiring[], unsigned fired[])
+{
+	for(unsigned g = start; g < end; ++g) {
+		fired[g] = fstim[g] ? 1u : 0u;
+		fstim[g] = 0;
+		recentFiring[g] = (recentFiring[g] << 1) | uint64_t(fired[g]);
+	}
+}
+
 static void
 update_if_curr_exp(unsigned start, unsigned end, bool with_noise,
 		const float* param, size_t pstride, float* state, size_t sstride,
@@ -421,7 +455,7 @@ struct FAxonTerminal { unsigned target; fix_t weight; };
 struct Oracle {
 	/* ---- network (src/nemo/NetworkImpl.cpp) */
 	std::vector<NeuronGroup> groups;
-	int typeOf[3];
+	int typeOf[NMODELS];
 	std::unordered_map<unsigned, std::pair<unsigned, unsigned> > netMapper;  // gidx -> (group, index)
 	/* synapses in insertion order (the per-source Axon vectors of
 	 * src/nemo/Axon.cpp:9-20, flattened); ids are (source << 32) | ordinal */
@@ -477,11 +511,11 @@ struct Oracle {
 
 	Oracle() : maxDelayNet(0), hasStdp(false), finalized(false), N(0), maxDelay(0),
 		cycle(0), wallMs(0) {
-		typeOf[0] = typeOf[1] = typeOf[2] = -1;
+		for(unsigned m = 0; m < NMODELS; ++m) typeOf[m] = -1;
 	}
 
 	unsigned groupFor(unsigned model) {
-		if(model > 2) throw exception(NEMO_INVALID_INPUT, "unknown neuron model");
+		if(model >= NMODELS) throw exception(NEMO_INVALID_INPUT, "unknown neuron model");
 		if(typeOf[model] < 0) {
 			typeOf[model] = int(groups.size());
 			NeuronGroup g;
@@ -764,6 +798,11 @@ struct Oracle {
 				update_izhikevich(start, end, &simParam[gi][0], n, &simState[gi][0], n,
 						&fstim[0], &simRng[gi][0], &curE[0], &curI[0], &curExt[0],
 						&recentFiring[0], &fired[0]);
+			} else if(model == MODEL_POISSON_SOURCE) {
+				update_poisson_source(start, end, &simParam[gi][0], &fstim[0], &simRng[gi][0],
+						&recentFiring[0], &fired[0]);
+			} else if(model == MODEL_INPUT) {
+				update_input(start, end, &fstim[0], &recentFiring[0], &fired[0]);
 			} else {
 				update_if_curr_exp(start, end, model == MODEL_IF_CURR_EXP_NOISE,
 						&simParam[gi][0], n, &simState[gi][0], n,
@@ -1035,7 +1074,11 @@ int oracle_get_state_all(void* o, unsigned var, float* out)
 {
 	ORACLE_CALL(
 		Oracle* s = static_cast<Oracle*>(o);
-		for(size_t l = 0; l < s->N; ++l) out[l] = s->getState(s->localToGlobal[l], var);
+		/* NaN for a neuron whose type has no state variable var */
+		for(size_t l = 0; l < s->N; ++l) {
+			out[l] = var < TYPES[s->groups[s->localGroup[l]].model].nstate
+				? s->getState(s->localToGlobal[l], var) : std::nanf("");
+		}
 	)
 }
 

@@ -18,7 +18,7 @@ _c_u8_p = ctypes.POINTER(ctypes.c_ubyte)
 _c_i32_p = ctypes.POINTER(ctypes.c_int32)
 _c_size_p = ctypes.POINTER(ctypes.c_size_t)
 
-MODEL_IDS = {"Izhikevich": 0, "IF_curr_exp": 1, "IF_curr_exp_noise": 2}
+MODEL_IDS = {"Izhikevich": 0, "IF_curr_exp": 1, "IF_curr_exp_noise": 2, "PoissonSource": 3, "Input": 4}
 
 _lib = None
 

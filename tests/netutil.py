@@ -68,3 +68,37 @@ def sat_to_float(v):
     clamp to int32 range, then float(v)/2^20 in float32."""
     v = max(min(v, 2**31 - 1), -2**31)
     return np.float32(np.float32(v) / np.float32(1 << 20))
+
+
+def mt19937(n):
+    """Reference MT19937 (Matsumoto & Nishimura), default seed 5489, first n
+    32-bit outputs, in Python integers."""
+    mt = [0] * 624
+    mt[0] = 5489
+    for i in range(1, 624):
+        mt[i] = (1812433253 * (mt[i - 1] ^ (mt[i - 1] >> 30)) + i) & 0xffffffff
+    idx = 624
+    out = []
+    for _ in range(n):
+        if idx >= 624:
+            for i in range(624):
+                y = (mt[i] & 0x80000000) | (mt[(i + 1) % 624] & 0x7fffffff)
+                mt[i] = mt[(i + 397) % 624] ^ (y >> 1) ^ (0x9908b0df if y & 1 else 0)
+            idx = 0
+        y = mt[idx]
+        idx += 1
+        y ^= y >> 11
+        y ^= (y << 7) & 0x9d2c5680
+        y ^= (y << 15) & 0xefc60000
+        y ^= y >> 18
+        out.append(y & 0xffffffff)
+    return out
+
+
+def xorshift128(s):
+    """One step of Marsaglia's xorshift128 on the state list s (updated in
+    place); returns the output."""
+    t = (s[0] ^ ((s[0] << 11) & 0xffffffff)) & 0xffffffff
+    s[0], s[1], s[2] = s[1], s[2], s[3]
+    s[3] = (s[3] ^ (s[3] >> 19)) ^ (t ^ (t >> 8))
+    return s[3]

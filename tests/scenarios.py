@@ -280,3 +280,35 @@ def stdp_in_flight(make, delay=20, reward=1.0, **sim_args):
     expect = np.float32(v_before + np.float32(np.float32(cur / np.float32(1.0)) + np.float32(0.0)))
     got = sim.get_neuron_state(X, 0)
     assert got == expect, (got, expect, v_before)
+
+
+def poisson_and_input(make, npois=64, ninput=16, cycles=300, seed=5):
+    """PoissonSource (src/nemo/cpu/plugins/PoissonSource.cpp:22-29) and Input
+    (src/nemo/cpu/plugins/Input.cpp:22-27) in closed form. Neuron k of the
+    Poisson group (simulation index k: it is the first type) draws from
+    xorshift128 seeded with MT19937 draws 4k..4k+3 shifted right by one
+    (src/nemo/RNG.cpp:38-63) and fires when the low 16 bits of its draw are
+    below rate * 2^16 -- rate 0 never, rate 1 always -- or when forced.
+    Input neurons fire exactly when forced; synapses into either kind (here
+    strong ones, from the Poisson sources) change nothing."""
+    b = make()
+    rng = np.random.default_rng(seed)
+    rates = np.concatenate([[0.0, 1.0, 0.5], rng.uniform(0.0, 0.2, npois - 3)]).astype(np.float32)
+    for k in range(npois):
+        b.add_neuron("PoissonSource", k, float(rates[k]))
+    for k in range(ninput):
+        b.add_neuron("Input", npois + k)
+    for k in range(npois):
+        for j in range(4):
+            b.add_synapse(k, int(rng.integers(0, npois + ninput)), int(rng.integers(1, 6)), 1000.0, 0)
+    sim = b.simulation()
+    draws = netutil.mt19937(4 * npois)
+    states = [[d >> 1 for d in draws[4 * k:4 * k + 4]] for k in range(npois)]
+    thresholds = [int(np.float32(r) * np.float32(65536.0)) for r in rates]
+    for t in range(cycles):
+        forced = sorted(rng.choice(npois + ninput, size=3, replace=False).tolist()) if t % 7 == 0 else []
+        expect = [k for k in range(npois)
+                  if (netutil.xorshift128(states[k]) & 0xffff) < thresholds[k] or k in forced]
+        expect += [k for k in forced if k >= npois]
+        got = sim.step(forced)
+        assert sorted(got) == sorted(expect), (t, sorted(got), sorted(expect))

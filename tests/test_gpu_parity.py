@@ -339,6 +339,40 @@ def test_device_construction_invalid(nemo_mod):
     assert msgs[0] == msgs[1] and "777" in msgs[0], msgs
 
 
+@pytest.mark.parametrize("step_mode", [None, "plugins"])
+def test_poisson_driven_network(nemo_mod, oracle_mod, step_mode):
+    """An Izhikevich 80/20 network driven by 300 PoissonSource neurons and 20
+    Input neurons (forced now and then), on the fused kernel (the any-model
+    variant) and on the plugin step: rasters and Izhikevich state bit-exact."""
+    from backends import HipBuilder, OracleBuilder
+    spec = examples.izhikevich_80_20(1000, 100, seed=61)
+    rng = np.random.default_rng(62)
+    syn = []
+    for k in range(320):
+        for _ in range(30):
+            syn.append((1000 + k, int(rng.integers(0, 1000)), int(rng.integers(1, 21)),
+                        float(rng.uniform(0.0, 2.0)), 0))
+    builders = [HipBuilder(nemo_mod), OracleBuilder(oracle_mod)]
+    for b in builders:
+        b.load_spec(spec)
+        for k in range(300):
+            b.add_neuron("PoissonSource", 1000 + k, 0.02)
+        for k in range(20):
+            b.add_neuron("Input", 1300 + k)
+        for s_, t_, d_, w_, p_ in syn:
+            b.add_synapse(s_, t_, d_, w_, p_)
+    sim = builders[0].simulation(step_mode=step_mode)
+    orc = builders[1].simulation()
+    assert sim.info()["fused"] == (0 if step_mode else 1)
+    _compare_run(sim, orc, 300)
+    for t in range(50):
+        fs = sorted(rng.choice(20, size=5, replace=False) + 1300) if t % 5 == 0 else []
+        fs = [int(x) for x in fs]
+        assert sim.step(fs) == orc.step(fs), t
+    _compare_run(sim, orc, 200)
+    _compare_state(sim, orc, 1000, 2, sample=range(0, 1000, 3))
+
+
 def test_synapse_queries(nemo_mod, oracle_mod):
     spec = examples.izhikevich_80_20(300, 20, seed=2)
     sim = _hip_sim(nemo_mod, spec)

@@ -61,3 +61,13 @@ def test_stdp_closed_form(make, reward, noise):
 @pytest.mark.parametrize("step_mode", ["fused", "plugins"])
 def test_stdp_in_flight(make, delay, step_mode):
     scenarios.stdp_in_flight(make, delay=delay, step_mode=step_mode)
+
+
+@pytest.mark.parametrize("step_mode", [None, "plugins"])
+def test_poisson_and_input(nemo_mod, step_mode):
+    from backends import HipBuilder
+
+    class Make(HipBuilder):
+        def simulation(self, partition_size=None, step_mode=step_mode):
+            return HipBuilder.simulation(self, partition_size, step_mode)
+    scenarios.poisson_and_input(lambda: Make(nemo_mod))

@@ -51,27 +51,7 @@ def test_mt19937_known_answer(oracle_mod):
 
 
 def _mt19937(n):
-    """Reference MT19937 (Matsumoto & Nishimura) in Python integers."""
-    mt = [0] * 624
-    mt[0] = 5489
-    for i in range(1, 624):
-        mt[i] = (1812433253 * (mt[i - 1] ^ (mt[i - 1] >> 30)) + i) & 0xffffffff
-    idx = 624
-    out = []
-    for _ in range(n):
-        if idx >= 624:
-            for i in range(624):
-                y = (mt[i] & 0x80000000) | (mt[(i + 1) % 624] & 0x7fffffff)
-                mt[i] = mt[(i + 397) % 624] ^ (y >> 1) ^ (0x9908b0df if y & 1 else 0)
-            idx = 0
-        y = mt[idx]
-        idx += 1
-        y ^= y >> 11
-        y ^= (y << 7) & 0x9d2c5680
-        y ^= (y << 15) & 0xefc60000
-        y ^= y >> 18
-        out.append(y & 0xffffffff)
-    return out
+    return netutil.mt19937(n)
 
 
 def test_initialise_rng_closed_form(oracle_mod):

@@ -79,3 +79,7 @@ def test_apply_stdp_without_stdp(oracle_mod):
 @pytest.mark.parametrize("delay", [20, 25])
 def test_stdp_in_flight(make, delay):
     scenarios.stdp_in_flight(make, delay=delay)
+
+
+def test_poisson_and_input(make):
+    scenarios.poisson_and_input(make)

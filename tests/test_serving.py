@@ -16,6 +16,7 @@ def test_budget_admission_order_cpu():
     assert [r.rid for r in sorted(reqs, key=lambda r: r.arrival_s)] == list(range(20))
     for r in reqs:
         assert r.estimated_device_bytes() > 8 * r.neurons * r.synapses
+        assert r.construction_bytes() > 30 * r.neurons * r.synapses
     too_big = serving.Request(0, 0.0, 50000, 1000, 10)
     srv = serving.Server(hbm_budget_bytes=10 << 20)
     with pytest.raises(ValueError):
@@ -30,8 +31,10 @@ def test_alone_equals_mix(oracle_mod):
         serving.Request(2, 0.01, 2500, 100, 1300, stdp=True, seed=33, stdp_period=500),
         serving.Request(3, 0.02, 1200, 60, 420, seed=34),
     ]
-    # a budget that holds only some of them at once, so admission is staggered
-    budget = reqs[0].estimated_device_bytes() + reqs[1].estimated_device_bytes() + reqs[2].estimated_device_bytes()
+    # a budget that holds only some of them at once (resident, plus the
+    # construction memory of one being built), so admission is staggered
+    budget = (reqs[0].estimated_device_bytes() + reqs[1].estimated_device_bytes() + reqs[2].estimated_device_bytes()
+              + max(r.construction_bytes() for r in reqs))
     srv = serving.Server(hbm_budget_bytes=budget, chunk_cycles=64, build_workers=2)
     stats = srv.serve(reqs, realtime=False)
     assert stats["requests"] == 4
