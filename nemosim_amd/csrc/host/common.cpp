@@ -60,10 +60,18 @@ addPluginPath(const std::string& dir)
 std::vector<std::string>
 pluginPaths()
 {
+	/* src/nemo/NeuronType.cpp:50-67: paths added by the program, then the
+	 * user directory ($HOME/.nemo/plugins), then the system directory (the
+	 * plugins/ directory beside this library) */
 	std::vector<std::string> p;
 	{
 		std::lock_guard<std::mutex> lock(g_modelMutex);
 		p = g_userPaths;
+	}
+	const char* home = getenv("HOME");
+	if(home && *home) {
+		const std::string user = std::string(home) + "/.nemo/plugins";
+		if(isDirectory(user)) p.push_back(user);
 	}
 	Dl_info info;
 	if(dladdr(reinterpret_cast<void*>(&pluginPaths), &info) && info.dli_fname) {

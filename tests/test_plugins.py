@@ -25,15 +25,18 @@ def _external_model_dir(tmp_path, name, template="Izhikevich"):
     return str(d)
 
 
+BUILTIN = ("Izhikevich", "IF_curr_exp", "IF_curr_exp_noise", "PoissonSource", "Input")
+
+
 def test_plugin_libraries_built():
-    for m in ("Izhikevich", "IF_curr_exp", "IF_curr_exp_noise"):
+    for m in BUILTIN:
         assert os.path.exists(os.path.join(PLUGINS, "hip", "lib%s.so" % m))
         assert os.path.exists(os.path.join(PLUGINS, m + ".ini"))
 
 
 def test_plugin_exports_abi_symbols():
     import ctypes
-    for m in ("Izhikevich", "IF_curr_exp", "IF_curr_exp_noise"):
+    for m in BUILTIN:
         lib = ctypes.CDLL(os.path.join(PLUGINS, "hip", "lib%s.so" % m))
         assert hasattr(lib, "hip_update_neurons") and hasattr(lib, "hip_init_neurons")
 
@@ -50,6 +53,54 @@ def test_external_descriptor_on_plugin_path(nemo_mod, tmp_path):
     with pytest.raises(nemo_mod.NemoError):
         net.add_neuron(t, 1, 0.02, 0.2, -65.0)        # descriptor: 5 + 2 arguments
     assert net.get_neuron_state(0, 1) == np.float32(-65.0)
+
+
+def test_user_plugin_directory(nemo_mod, tmp_path, monkeypatch):
+    """$HOME/.nemo/plugins is searched after the program's own paths and
+    before the system directory (src/nemo/NeuronType.cpp:50-67)."""
+    d = _external_model_dir(tmp_path, "HomeIzhikevich")
+    home = tmp_path / "home"
+    (home / ".nemo").mkdir(parents=True)
+    shutil.move(d, str(home / ".nemo" / "plugins"))
+    with pytest.raises(nemo_mod.NemoError):
+        nemo_mod.Network().add_neuron_type("HomeIzhikevich")
+    monkeypatch.setenv("HOME", str(home))
+    net = nemo_mod.Network()
+    t = net.add_neuron_type("HomeIzhikevich")
+    net.add_neuron(t, 0, 0.02, 0.2, -65.0, 8.0, 5.0, -13.0, -65.0)
+
+
+def test_info_tool(tmp_path):
+    """python -m nemosim_amd: the counterpart of NeMo's nemo utility
+    (src/app/nemo.cpp:46-98)."""
+    import subprocess
+    import sys
+
+    def run(*args):
+        env = dict(os.environ, HOME=str(tmp_path))
+        return subprocess.run([sys.executable, "-m", "nemosim_amd"] + list(args), cwd=ROOT, env=env,
+                              capture_output=True, text=True)
+    r = run("--version")
+    assert r.returncode == 0 and r.stdout.strip().startswith("0.7.2")
+    r = run("--system-plugin-path")
+    assert r.returncode == 0 and os.path.samefile(r.stdout.strip(), PLUGINS)
+    r = run("--user-plugin-path")
+    assert r.returncode == 0 and r.stdout.strip() == str(tmp_path / ".nemo" / "plugins")
+    r = run("--help")
+    assert r.returncode == 0 and "--list-devices" in r.stdout
+    r = run()
+    assert r.returncode == 1 and "--list-devices" in r.stdout
+    r = run("--no-such-option")
+    assert r.returncode == 1
+
+
+@pytest.mark.gpu
+def test_info_tool_lists_devices():
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, "-m", "nemosim_amd", "-l"], cwd=ROOT, capture_output=True, text=True)
+    assert r.returncode == 0
+    assert r.stdout.startswith("0: "), r.stdout
 
 
 def test_plugin_path_must_exist(nemo_mod, tmp_path):
