@@ -146,7 +146,7 @@ def bench_plain(args, stdp=False):
         pre, post = examples.stdp_window()
         stdp_fn = (pre, post, 0.0, 1.0, 0.0, -1.0)
     t0 = time.perf_counter()
-    sim = build_sim(nemosim_amd, spec, stdp=stdp_fn)
+    sim = build_sim(nemosim_amd, spec, stdp=stdp_fn, partition_size=args.partition_size or None)
     info = sim.info()
     log("  simulation built in %.1f s: %s" % (time.perf_counter() - t0, info))
     L = max(1, min(args.launch_steps, args.steps))
@@ -327,6 +327,7 @@ def main():
     ap.add_argument("--launch-steps", type=int, default=1000, help="cycles per persistent launch")
     ap.add_argument("--neurons", type=int, default=200000)
     ap.add_argument("--synapses", type=int, default=1000)
+    ap.add_argument("--partition-size", type=int, default=0, help="neurons per workgroup (0: automatic)")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
