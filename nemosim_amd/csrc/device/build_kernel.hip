@@ -104,8 +104,10 @@ __global__ void build_scatter_kernel(const uint32_t* lsrc, const uint32_t* ltgt,
 	}
 }
 
+/* one atomic per block (a grid of at most 1024 blocks) */
 __global__ void build_max_kernel(const uint32_t* v, size_t n, uint32_t* out)
 {
+	__shared__ uint32_t wmax[BLOCK / 64];
 	uint32_t m = 0;
 	for(size_t i = size_t(blockIdx.x) * BLOCK + threadIdx.x; i < n; i += size_t(gridDim.x) * BLOCK) {
 		m = v[i] > m ? v[i] : m;
@@ -114,7 +116,12 @@ __global__ void build_max_kernel(const uint32_t* v, size_t n, uint32_t* out)
 		const uint32_t y = __shfl_xor(m, d, 64);
 		m = y > m ? y : m;
 	}
-	if((threadIdx.x & 63u) == 0u) atomicMax(out, m);
+	if((threadIdx.x & 63u) == 0u) wmax[threadIdx.x >> 6] = m;
+	__syncthreads();
+	if(threadIdx.x == 0) {
+		for(int w = 1; w < BLOCK / 64; ++w) m = wmax[w] > m ? wmax[w] : m;
+		atomicMax(out, m);
+	}
 }
 
 __global__ void build_near_pack_kernel(uint32_t* off, const uint32_t* nearCount, size_t nb, uint32_t shift)
@@ -194,7 +201,8 @@ int nemo_dev_build_scatter(const uint32_t* lsrc, const uint32_t* ltgt, const uin
 
 int nemo_dev_build_max(const uint32_t* v, size_t n, uint32_t* out, void* stream)
 {
-	hipLaunchKernelGGL(build_max_kernel, dim3(grid_for(n)), dim3(BLOCK), 0, static_cast<hipStream_t>(stream),
+	const unsigned g = grid_for(n);
+	hipLaunchKernelGGL(build_max_kernel, dim3(g < 1024u ? g : 1024u), dim3(BLOCK), 0, static_cast<hipStream_t>(stream),
 			v, n, out);
 	return int(hipGetLastError());
 }
