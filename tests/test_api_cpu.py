@@ -63,6 +63,28 @@ def test_network_construction_and_queries(nemo_mod):
     assert net.get_neuron_parameter(3, 0) == np.float32(0.1)
 
 
+@pytest.mark.parametrize("start,step", [(0, 1), (1000, 1), (0, 5), (1000, 5)])
+def test_neuron_count(nemo_mod, start, step):
+    """src/test/test.cpp:150-174: neuronCount counts neurons, whatever their
+    indices (contiguous or not, from zero or not)."""
+    net = nemo_mod.Network()
+    iz = net.add_neuron_type("Izhikevich")
+    for i in range(2000):
+        net.add_neuron(iz, start + i * step, 0.1, 0.2, 0.3, 0.4, 0.5, 0.6, 0.7)
+    assert net.neuron_count() == 2000
+
+
+def test_invalid_neuron_type(nemo_mod):
+    """src/test/test.cpp:966-977: a type id that was never returned by
+    add_neuron_type is rejected."""
+    net = nemo_mod.Network()
+    iz = net.add_neuron_type("Izhikevich")
+    for t in range(100):
+        if t != iz:
+            with pytest.raises(nemo_mod.NemoError):
+                net.add_neuron(t, iz, 0, 0, 0, 0, 0, 0, 0)
+
+
 def test_bulk_and_single_synapse_ids_agree(nemo_mod):
     net = nemo_mod.Network()
     iz = net.add_neuron_type("Izhikevich")

@@ -71,3 +71,17 @@ def test_poisson_and_input(nemo_mod, step_mode):
         def simulation(self, partition_size=None, step_mode=step_mode):
             return HipBuilder.simulation(self, partition_size, step_mode)
     scenarios.poisson_and_input(lambda: Make(nemo_mod))
+
+
+def test_no_parameter_neuron_type(nemo_mod):
+    """src/test/test.cpp:981-1001 (there with IzhikevichRS, a test model of
+    NeMo's CUDA backend): a type without parameters, here Input, simulates;
+    its neurons fire exactly when forced."""
+    net = nemo_mod.Network()
+    t = net.add_neuron_type("Input")
+    for n in range(1000):
+        net.add_neuron(t, n)
+    sim = nemo_mod.Simulation(net, nemo_mod.Configuration())
+    for c in range(100):
+        forced = [c, 999 - c] if c % 3 == 0 else []
+        assert sorted(sim.step(forced)) == sorted(set(forced))
