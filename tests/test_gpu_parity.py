@@ -452,6 +452,30 @@ def test_delivery_lanes(nemo_mod, oracle_mod, lanes):
     _compare_run(sim, orc, 100)
 
 
+def test_batch_coresidency_lds_rounding(nemo_mod):
+    """A batch whose workgroups fit the device only by the occupancy query's
+    count: 12 networks of 100 partitions of 192 neurons (about 32.5 KB of LDS
+    each, which the query fits 5 to a CU but the device only 4) must be split
+    into launches of workgroups that are all resident, and give each
+    network's own raster."""
+    specs = [examples.izhikevich_80_20(19200, 50, seed=200 + i) for i in range(12)]
+    sims, alone = [], []
+    for s in specs:
+        for target in (sims, alone):
+            conf = nemo_mod.Configuration()
+            conf.set_partition_size(192)
+            target.append(nemo_mod.Simulation(s.to_nemo(nemo_mod), conf))
+    lds = sims[0].info()["lds_bytes"]
+    assert sims[0].info()["partitions"] == 100 and 32 * 1024 < lds + 512 <= 33 * 1024, lds
+    nemo_mod.run_batch(sims, [200] * len(sims))
+    for k, (b, a) in enumerate(zip(sims, alone)):
+        a.run(200)
+        bc, bi = b.read_fired()
+        ac, ai = a.read_fired()
+        d = first_divergence(ac, ai, bc, bi)
+        assert d is None, "network %d: first diverging cycle %d: %s" % ((k,) + d)
+
+
 def test_batch_graph_replays(nemo_mod, oracle_mod):
     """The batched launch is an instantiated HIP graph replayed once per
     call: repeated calls on the same pack (same graph), a changed pack (a
