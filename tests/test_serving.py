@@ -24,7 +24,8 @@ def test_budget_admission_order_cpu():
 
 
 @pytest.mark.gpu
-def test_alone_equals_mix(oracle_mod):
+@pytest.mark.parametrize("device_construction", [False, True])
+def test_alone_equals_mix(oracle_mod, device_construction):
     reqs = [
         serving.Request(0, 0.00, 1500, 100, 350, seed=31),
         serving.Request(1, 0.00, 4000, 80, 250, model="IF_curr_exp_noise", seed=32),
@@ -35,7 +36,8 @@ def test_alone_equals_mix(oracle_mod):
     # construction memory of one being built), so admission is staggered
     budget = (reqs[0].estimated_device_bytes() + reqs[1].estimated_device_bytes() + reqs[2].estimated_device_bytes()
               + max(r.construction_bytes() for r in reqs))
-    srv = serving.Server(hbm_budget_bytes=budget, chunk_cycles=64, build_workers=2)
+    srv = serving.Server(hbm_budget_bytes=budget, chunk_cycles=64, build_workers=2,
+                         device_construction=device_construction)
     stats = srv.serve(reqs, realtime=False)
     assert stats["requests"] == 4
     assert stats["p99_latency_s"] >= stats["p50_latency_s"] > 0
