@@ -95,6 +95,62 @@ def if_curr_exp_80_20(n, m, seed, dmax=20, stdp=False, sigma=3.0, noise=True):
     return NetSpec(groups, rowptr, targets, delays, weights, plastic)
 
 
+TORUS_PATCH = 32           # patch width and height (neurons)
+TORUS_MAX_DELAY = 20
+
+
+def torus(pcount, m, seed, stdp=False, sigma=16.0):
+    """The toroidal network of src/examples/torus.cpp:216-286, the network of
+    NeMo's high-firing test: pcount patches of 32 x 32 neurons side by side
+    along x (neuron patch*1024 + y*32 + x), each neuron excitatory with
+    probability 0.8. Excitatory: a=0.02, b=0.2, c=-65+15 r1^2, d=8-6 r2^2,
+    sigma=5; inhibitory: a=0.02+0.08 r1, b=0.25-0.05 r2, c=-65, d=2, sigma=2;
+    v0=-65, u0=b*v0 (:68-96). Each neuron has m synapses to the neuron at
+    distance 1+|N(0, s)| (s = sigma excitatory, 16 inhibitory) in a uniform
+    direction, offsets rounded half away from zero, wrapping around the torus
+    (:110-150); delay 1 + distance/32 from the truncated distance, at most 20
+    (:153-163); weight 0.5 U (excitatory, plastic when stdp) or -U
+    (inhibitory, static). The draws come from numpy's PCG64, not Boost's
+    MT19937: the same shape of network, not the same network."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    W = TORUS_PATCH
+    n = pcount * W * W
+    exc = rng.random(n) < 0.8
+    r1 = rng.random(n).astype(np.float32)
+    r2 = rng.random(n).astype(np.float32)
+    v = np.full(n, -65.0, np.float32)
+    a = np.where(exc, np.float32(0.02), np.float32(0.02) + np.float32(0.08) * r1).astype(np.float32)
+    b = np.where(exc, np.float32(0.2), np.float32(0.25) - np.float32(0.05) * r2).astype(np.float32)
+    c = np.where(exc, v + np.float32(15.0) * r1 * r1, v).astype(np.float32)
+    d = np.where(exc, np.float32(8.0) - np.float32(6.0) * r2 * r2, np.float32(2.0)).astype(np.float32)
+    sig = np.where(exc, np.float32(5.0), np.float32(2.0)).astype(np.float32)
+    u = (b * v).astype(np.float32)
+    args = np.stack([a, b, c, d, sig, u, v], axis=1).astype(np.float32)
+    groups = [(IZHIKEVICH, np.arange(n, dtype=np.uint32), args)]
+
+    src = np.repeat(np.arange(n, dtype=np.int64), m)
+    src_exc = exc[src]
+    scale = np.where(src_exc, float(sigma), W / 2.0)
+    dist = 1.0 + np.abs(rng.normal(0.0, 1.0, size=n * m) * scale)
+    theta = rng.uniform(0.0, 2.0 * np.pi, size=n * m)
+
+    def away(x):
+        return np.where(x > 0.0, np.floor(x + 0.5), np.ceil(x - 0.5)).astype(np.int64)
+
+    patch, rem = np.divmod(src, W * W)
+    sy, sx = np.divmod(rem, W)
+    gx = np.mod(patch * W + sx + away(dist * np.cos(theta)), W * pcount)
+    ty = np.mod(sy + away(dist * np.sin(theta)), W)
+    targets = ((gx // W) * W * W + ty * W + gx % W).astype(np.uint32)
+    idist = dist.astype(np.int64)
+    delays = np.where(idist >= TORUS_MAX_DELAY * W, TORUS_MAX_DELAY, 1 + idist // W).astype(np.uint8)
+    uw = rng.random(n * m, dtype=np.float32)
+    weights = np.where(src_exc, np.float32(0.5) * uw, -uw).astype(np.float32)
+    plastic = (src_exc & bool(stdp)).astype(np.uint8)
+    rowptr = np.arange(0, n * m + 1, m, dtype=np.uint64)
+    return NetSpec(groups, rowptr, targets, delays, weights, plastic)
+
+
 def stdp_window(pre_amp=0.1, post_amp=-0.12, tau=20.0, length=20):
     """Exponential STDP window sampled at dt = 1..length
     (src/examples/common.cpp:154-165 sampling convention)."""

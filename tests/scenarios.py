@@ -11,13 +11,13 @@ IZ = "Izhikevich"
 IF = "IF_curr_exp"
 
 
-def create_ring(b, ncount, n0=0, delay=1, plastic=0):
-    """src/test/utils.cpp:133-145"""
+def create_ring(b, ncount, n0=0, delay=1, plastic=0, nstep=1):
+    """src/test/utils.cpp:125-145: neuron k of the ring has index n0 + k*nstep"""
     for i in range(ncount):
         v, bb, r2 = -65.0, 0.2, 0.25
-        src = n0 + i
+        src = n0 + i * nstep
         b.add_neuron(IZ, src, 0.02, bb, v + 15.0 * r2, 8.0 - 6.0 * r2, 0.0, np.float32(bb) * np.float32(v), v)
-        b.add_synapse(src, n0 + (i + 1) % ncount, delay, 1000.0, plastic)
+        b.add_synapse(src, n0 + ((i + 1) % ncount) * nstep, delay, 1000.0, plastic)
 
 
 def ring(make, ncount, delay, partition_size=None):
@@ -312,3 +312,77 @@ def poisson_and_input(make, npois=64, ninput=16, cycles=300, seed=5):
         expect += [k for k in forced if k >= npois]
         got = sim.step(forced)
         assert sorted(got) == sorted(expect), (t, sorted(got), sorted(expect))
+
+
+def non_contiguous_indices(make, n0, nstep, ncount=1000, seconds=2):
+    """src/test/test.cpp:517-548: a ring whose indices start at n0 and are
+    nstep apart fires as the same ring from index 0 does, one neuron per
+    cycle, with every index offset by n0."""
+    runs = []
+    for first in (0, n0):
+        b = make()
+        create_ring(b, ncount, n0=first, nstep=nstep)
+        sim = b.simulation()
+        fired = [sim.step([first])]
+        fired += [sim.step() for _ in range(seconds * 1000 - 1)]
+        assert sum(len(f) for f in fired) == seconds * ncount
+        runs.append(fired)
+    for t, (f0, f1) in enumerate(zip(*runs)):
+        assert [x + n0 for x in f0] == f1, (t, f0, f1)
+
+
+def reference_stdp_config():
+    """The STDP function of NeMo's test configuration (src/test/utils.cpp:84-97):
+    pre 0.1 exp(-dt/20), post -0.08 exp(-dt/20) for dt = 1..20, weights in
+    [-10, 10] through the four-argument form (minE=0, maxE=10, minI=0,
+    maxI=-10, src/nemo/Configuration.cpp:81-88)."""
+    dt = np.arange(1, 21, dtype=np.float32)
+    pre = (np.float32(0.1) * np.exp(-dt / np.float32(20.0))).astype(np.float32)
+    post = (np.float32(-0.08) * np.exp(-dt / np.float32(20.0))).astype(np.float32)
+    return pre, post, 0.0, 10.0, 0.0, -10.0
+
+
+def no_outgoing(make, ncount, first, last, contiguous, stdp):
+    """src/test/networks.cpp:15-78: ncount excitatory neurons (indices i, or
+    50 + 3i), all-to-all synapses of delay 1 and weight 2 except from the
+    neurons [first, last), which have none; neurons with index / 8 == t are
+    forced to fire at cycle t, for 20 cycles. NeMo only asks that this runs;
+    the fired lists and, with STDP, the weights after applyStdp(1.0) are
+    returned for a comparison between backends. With STDP the run is 60
+    cycles, so that the postfire window of the first firings has closed and
+    every weight has moved."""
+    cycles = 60 if stdp else 20
+    def index(i):
+        return i if contiguous else 50 + 3 * i
+    b = make()
+    if stdp:
+        b.set_stdp(*reference_stdp_config())
+    sids = []
+    for i in range(ncount):
+        add_excitatory_neuron(b, index(i))
+        if not (first <= i < last):
+            for j in range(ncount):
+                sids.append(b.add_synapse(index(i), index(j), 1, 2.0, int(stdp)))
+    sim = b.simulation()
+    fired = []
+    for t in range(cycles):
+        fired.append(sim.step([index(i) for i in range(ncount) if index(i) // 8 == t]))
+    weights = []
+    if stdp:
+        sim.apply_stdp(1.0)
+        weights = [sim.get_synapse_weight(sid) for sid in sids]
+    return fired, weights
+
+
+def simple_network(make, start, count, cycles=100):
+    """src/test/test.cpp:138-148 (unary network, network without synapses,
+    one-based indices): count Izhikevich neurons with indices start.. and
+    NeMo's placeholder parameters (a..d = 0.1..0.4, u=0.5, v=0.6, sigma=0.7)
+    and no synapses step without error. The fired lists and the final
+    membrane potentials are returned for a comparison between backends."""
+    b = make()
+    for k in range(count):
+        b.add_neuron(IZ, start + k, 0.1, 0.2, 0.3, 0.4, 0.7, 0.5, 0.6)
+    sim = b.simulation()
+    fired = [sim.step() for _ in range(cycles)]
+    return fired, [sim.get_membrane_potential(start + k) for k in range(count)]

@@ -7,46 +7,10 @@ import pytest
 
 import nrand_domain as dom
 from backends import first_divergence
+from parity import compare_run, compare_state, compare_weights, hip_sim, same_float
 from nemosim_amd import examples
 
 pytestmark = pytest.mark.gpu
-
-
-def _hip_sim(nemo_mod, spec, stdp=None, partition_size=None):
-    net = spec.to_nemo(nemo_mod)
-    conf = nemo_mod.Configuration()
-    if stdp is not None:
-        pre, post, min_e, max_e, min_i, max_i = stdp
-        conf.set_stdp_function(pre, post, min_e=min_e, max_e=max_e, min_i=min_i, max_i=max_i)
-    if partition_size:
-        conf.set_partition_size(partition_size)
-    return nemo_mod.Simulation(net, conf)
-
-
-def _same_float(a, b):
-    a = np.asarray(a, np.float32)
-    b = np.asarray(b, np.float32)
-    both_nan = np.isnan(a) & np.isnan(b)
-    return np.all((a.view(np.uint32) == b.view(np.uint32)) | both_nan)
-
-
-def _compare_state(sim, orc, n, nstate, sample=None):
-    idx = range(n) if sample is None else sample
-    for var in range(nstate):
-        ref = orc.state_all(var)
-        got = np.array([sim.get_neuron_state(i, var) for i in idx], np.float32)
-        exp = ref[list(idx)]
-        assert _same_float(got, exp), "state %d differs at neurons %s" % (
-            var, np.nonzero(got.view(np.uint32) != exp.view(np.uint32))[0][:10])
-
-
-def _compare_run(sim, orc, steps):
-    sim.run(steps)
-    gc, gi = sim.read_fired()
-    rc, ri = orc.run(steps)
-    d = first_divergence(rc, ri, gc, gi)
-    assert d is None, "first diverging cycle %d: %s" % d
-    return rc, ri
 
 
 def test_device_libm_exhaustive(nemo_mod, oracle_mod):
@@ -71,23 +35,23 @@ def test_device_nrand_sequence(nemo_mod, oracle_mod):
     for state in ([1, 2, 3, 4], [0x7fffffff, 0x12345678, 0x0badf00d, 0x3eadbeef], [5489, 0, 0, 1]):
         ref = oracle_mod.nrand_seq(state, 4000)
         got = nemo_mod.nrand_seq(state, 4000)
-        assert _same_float(got, ref)
+        assert same_float(got, ref)
 
 
 @pytest.mark.parametrize("ps", [None, 64, 256])
 def test_c1_raster_and_state(nemo_mod, oracle_mod, ps):
     """Izhikevich 80/20, 1000 neurons x 100 synapses, 1000 cycles."""
     spec = examples.izhikevich_80_20(1000, 100, seed=11)
-    sim = _hip_sim(nemo_mod, spec, partition_size=ps)
+    sim = hip_sim(nemo_mod, spec, partition_size=ps)
     orc = oracle_mod.from_spec(spec)
-    rc, _ = _compare_run(sim, orc, 1000)
+    rc, _ = compare_run(sim, orc, 1000)
     assert rc.sum() > 1000
-    _compare_state(sim, orc, 1000, 2)
+    compare_state(sim, orc, 1000, 2)
 
 
 def test_step_api_equals_run_api(nemo_mod, oracle_mod):
     spec = examples.izhikevich_80_20(1000, 100, seed=5)
-    sim = _hip_sim(nemo_mod, spec)
+    sim = hip_sim(nemo_mod, spec)
     orc = oracle_mod.from_spec(spec)
     rng = np.random.default_rng(2)
     for t in range(300):
@@ -97,13 +61,13 @@ def test_step_api_equals_run_api(nemo_mod, oracle_mod):
         b = orc.step(fs, ist)
         assert a == b, t
     # then continue with a device-resident run
-    _compare_run(sim, orc, 500)
-    _compare_state(sim, orc, 1000, 2)
+    compare_run(sim, orc, 500)
+    compare_state(sim, orc, 1000, 2)
 
 
 def test_run_with_schedule(nemo_mod, oracle_mod):
     spec = examples.izhikevich_80_20(2000, 100, seed=8)
-    sim = _hip_sim(nemo_mod, spec)
+    sim = hip_sim(nemo_mod, spec)
     orc = oracle_mod.from_spec(spec)
     rng = np.random.default_rng(4)
     steps = 200
@@ -122,22 +86,22 @@ def test_run_with_schedule(nemo_mod, oracle_mod):
 
 def test_if_curr_exp_noise_parity(nemo_mod, oracle_mod):
     spec = examples.if_curr_exp_80_20(3000, 100, seed=3)
-    sim = _hip_sim(nemo_mod, spec)
+    sim = hip_sim(nemo_mod, spec)
     orc = oracle_mod.from_spec(spec)
-    rc, _ = _compare_run(sim, orc, 800)
+    rc, _ = compare_run(sim, orc, 800)
     assert rc.sum() > 0
-    _compare_state(sim, orc, 3000, 4)
+    compare_state(sim, orc, 3000, 4)
 
 
 def test_if_curr_exp_plain_parity(nemo_mod, oracle_mod):
     spec = examples.if_curr_exp_80_20(2000, 100, seed=4, noise=False)
-    sim = _hip_sim(nemo_mod, spec)
+    sim = hip_sim(nemo_mod, spec)
     orc = oracle_mod.from_spec(spec)
     rng = np.random.default_rng(1)
     for t in range(300):
         ist = [(int(i), 20.0) for i in rng.integers(0, 2000, size=30)]
         assert sim.step(None, ist) == orc.step(None, ist), t
-    _compare_state(sim, orc, 2000, 4)
+    compare_state(sim, orc, 2000, 4)
 
 
 def test_mixed_types(nemo_mod, oracle_mod):
@@ -155,9 +119,9 @@ def test_mixed_types(nemo_mod, oracle_mod):
     delays = rng.integers(1, 21, size=n * m, dtype=np.uint8)
     weights = np.concatenate([a.weights, b.weights]).astype(np.float32)
     spec = examples.NetSpec(groups, rowptr, targets, delays, weights, np.zeros(n * m, np.uint8))
-    sim = _hip_sim(nemo_mod, spec)
+    sim = hip_sim(nemo_mod, spec)
     orc = oracle_mod.from_spec(spec)
-    _compare_run(sim, orc, 600)
+    compare_run(sim, orc, 600)
 
 
 def test_stdp_parity(nemo_mod, oracle_mod):
@@ -167,10 +131,10 @@ def test_stdp_parity(nemo_mod, oracle_mod):
     spec = examples.izhikevich_80_20(2000, 100, seed=13, stdp=True)
     pre, post = examples.stdp_window()
     stdp = (pre, post, 0.0, 1.0, 0.0, -1.0)
-    sim = _hip_sim(nemo_mod, spec, stdp=stdp)
+    sim = hip_sim(nemo_mod, spec, stdp=stdp)
     orc = oracle_mod.from_spec(spec, stdp=stdp)
     for epoch in range(5):
-        _compare_run(sim, orc, 100)
+        compare_run(sim, orc, 100)
         sim.apply_stdp(1.0)
         orc.apply_stdp(1.0)
     ref_w = orc.weights_fx(2000, spec.nsyn)
@@ -182,17 +146,6 @@ def test_stdp_parity(nemo_mod, oracle_mod):
         exp = np.float32(np.float32(ref_w[k]) / np.float32(1 << 20))
         assert got == exp, (k, got, exp)
     assert np.any(ref_w != np.array([int(np.trunc(np.float64(w) * (1 << 20))) for w in spec.weights], np.int64))
-
-
-def _compare_weights(sim, orc, spec, nsample, seed=0):
-    ref_w = orc.weights_fx(spec.n, spec.nsyn)
-    sample = np.random.default_rng(seed).integers(0, spec.nsyn, size=nsample)
-    for k in sample:
-        src = int(np.searchsorted(spec.rowptr, k, side="right") - 1)
-        sid = (src << 32) | int(k - spec.rowptr[src])
-        got = sim.get_synapse_weight(sid)
-        exp = np.float32(np.float32(ref_w[k]) / np.float32(1 << 20))
-        assert got == exp, (k, got, exp)
 
 
 @pytest.mark.parametrize("staging", [True, False])
@@ -218,16 +171,16 @@ def test_stdp_work_lists(nemo_mod, oracle_mod, staging):
     assert info["stdp_prefix_ring"] == (1 if staging else 0), info
     assert (info["stdp_staged_chunks"] > 0) == staging, info
     orc = oracle_mod.from_spec(spec, stdp=(pre, post, 0.0, 1.0, 0.0, -1.0))
-    _compare_run(sim, orc, 50)
+    compare_run(sim, orc, 50)
     rng = np.random.default_rng(8)
     for t in range(60):
         fs = sorted(rng.choice(n, size=3000, replace=False).tolist()) if t % 30 == 0 else []
         assert sim.step(fs) == orc.step(fs), t
-    _compare_run(sim, orc, 100)
+    compare_run(sim, orc, 100)
     sim.apply_stdp(1.0)
     orc.apply_stdp(1.0)
-    _compare_run(sim, orc, 50)
-    _compare_weights(sim, orc, spec, 3000)
+    compare_run(sim, orc, 50)
+    compare_weights(sim, orc, spec, 3000)
 
 
 def _construction_pair(nemo_mod, fill, stdp=None, step_mode=None):
@@ -364,18 +317,18 @@ def test_poisson_driven_network(nemo_mod, oracle_mod, step_mode):
     sim = builders[0].simulation(step_mode=step_mode)
     orc = builders[1].simulation()
     assert sim.info()["fused"] == (0 if step_mode else 1)
-    _compare_run(sim, orc, 300)
+    compare_run(sim, orc, 300)
     for t in range(50):
         fs = sorted(rng.choice(20, size=5, replace=False) + 1300) if t % 5 == 0 else []
         fs = [int(x) for x in fs]
         assert sim.step(fs) == orc.step(fs), t
-    _compare_run(sim, orc, 200)
-    _compare_state(sim, orc, 1000, 2, sample=range(0, 1000, 3))
+    compare_run(sim, orc, 200)
+    compare_state(sim, orc, 1000, 2, sample=range(0, 1000, 3))
 
 
 def test_synapse_queries(nemo_mod, oracle_mod):
     spec = examples.izhikevich_80_20(300, 20, seed=2)
-    sim = _hip_sim(nemo_mod, spec)
+    sim = hip_sim(nemo_mod, spec)
     orc = oracle_mod.from_spec(spec)
     for src in [0, 17, 299]:
         ids = sim.get_synapses_from(src)
@@ -393,7 +346,7 @@ def test_batch_equals_alone(nemo_mod, oracle_mod):
     equals its own oracle run."""
     sizes = [700, 1500, 3100, 1000, 5000]
     specs = [examples.izhikevich_80_20(n, 100, seed=100 + i) for i, n in enumerate(sizes)]
-    sims = [_hip_sim(nemo_mod, s) for s in specs]
+    sims = [hip_sim(nemo_mod, s) for s in specs]
     steps = [400, 300, 400, 250, 400]
     nemo_mod.run_batch(sims, steps)
     for s, sim, k in zip(specs, sims, steps):
@@ -424,8 +377,8 @@ def test_deferred_and_immediate_delivery(nemo_mod, oracle_mod, dense):
     info = sim.info()
     assert info["fused"] == 1
     assert info["deferred"] == (0 if dense else 1), info
-    _compare_run(sim, orc, 400)
-    _compare_state(sim, orc, spec.n, 2, sample=range(0, spec.n, 7))
+    compare_run(sim, orc, 400)
+    compare_state(sim, orc, spec.n, 2, sample=range(0, spec.n, 7))
 
 
 @pytest.mark.parametrize("lanes", [1, 2, 16, 64])
@@ -442,14 +395,14 @@ def test_delivery_lanes(nemo_mod, oracle_mod, lanes):
     sim = nemo_mod.Simulation(net, conf)
     assert sim.info()["lanes_per_source"] == lanes
     orc = oracle_mod.from_spec(spec)
-    _compare_run(sim, orc, 200)
+    compare_run(sim, orc, 200)
     # bursts: 1200 forced sources in one cycle put far more than 63 long
     # buckets into each wave's batch
     rng = np.random.default_rng(5)
     for t in range(40):
         fs = sorted(rng.choice(2500, size=1200, replace=False).tolist()) if t % 10 == 0 else []
         assert sim.step(fs) == orc.step(fs), t
-    _compare_run(sim, orc, 100)
+    compare_run(sim, orc, 100)
 
 
 def test_batch_coresidency_lds_rounding(nemo_mod):
@@ -484,7 +437,7 @@ def test_batch_graph_replays(nemo_mod, oracle_mod):
     potentials equal its own oracle run."""
     sizes = [900, 2100, 1300]
     specs = [examples.izhikevich_80_20(n, 100, seed=300 + i) for i, n in enumerate(sizes)]
-    sims = [_hip_sim(nemo_mod, s) for s in specs]
+    sims = [hip_sim(nemo_mod, s) for s in specs]
     orcs = [oracle_mod.from_spec(s) for s in specs]
     chunks = [[50, 50, 50], [50, 50, 50], [30, 70, 50], [60, 0, 40], [50, 50, 50], [50, 50, 50]]
     for c, ch in enumerate(chunks):
@@ -497,7 +450,53 @@ def test_batch_graph_replays(nemo_mod, oracle_mod):
             d = first_divergence(rc, ri, gc, gi)
             assert d is None, "chunk %d network %d: first diverging cycle %d: %s" % ((c, i) + d)
             for g in (0, spec.n // 2, spec.n - 1):
-                assert _same_float(sim.get_membrane_potential(g), orc.get_membrane_potential(g)), (c, i, g)
+                assert same_float(sim.get_membrane_potential(g), orc.get_membrane_potential(g)), (c, i, g)
+
+
+@pytest.mark.parametrize("partition_size", [None, 128])
+def test_high_firing(nemo_mod, oracle_mod, partition_size):
+    """src/test/test.cpp:697-730: the torus network (one 32 x 32 patch,
+    m = 1000, sigma = 16) with every neuron forced to fire every cycle, so
+    every partition's fired list is full. Most synapses have delay 1
+    (targets within 32 neurons), so buckets hold hundreds of delay-1 records
+    and delivery is immediate. 300 forced cycles, then 100 free ones, whose
+    rasters and the final state carry the currents still in flight from the
+    forced phase: bit-exact against the oracle. Then the network is forced
+    for 600 more cycles (NeMo's test runs 1000)."""
+    spec = examples.torus(1, 1000, seed=3)
+    sim = hip_sim(nemo_mod, spec, partition_size=partition_size)
+    info = sim.info()
+    assert info["deferred"] == 0, info
+    orc = oracle_mod.from_spec(spec)
+    everyone = list(range(spec.n))
+    for t in range(300):
+        got = sim.step(everyone)
+        assert got == everyone, t
+        assert orc.step(everyone) == everyone, t
+    for t in range(100):
+        assert sim.step() == orc.step(), t
+    compare_state(sim, orc, spec.n, 2)
+    for t in range(600):
+        assert sim.step(everyone) == everyone, t
+
+
+def test_stdp_all_static(nemo_mod, oracle_mod):
+    """src/test/test.cpp:661-673: STDP configured but every synapse static
+    (1000 neurons x 1000 synapses, delay 1), four epochs of 1000 cycles each
+    followed by applyStdp(1.0): rasters bit-exact, weights unchanged."""
+    import scenarios
+    spec = examples.izhikevich_80_20(1000, 1000, seed=17, dmax=1)
+    stdp = scenarios.reference_stdp_config()
+    sim = hip_sim(nemo_mod, spec, stdp=stdp)
+    assert sim.info()["plastic_synapses"] == 0
+    orc = oracle_mod.from_spec(spec, stdp=stdp)
+    for epoch in range(4):
+        compare_run(sim, orc, 1000)
+        sim.apply_stdp(1.0)
+        orc.apply_stdp(1.0)
+    compare_weights(sim, orc, spec, 2000)
+    initial = np.array([int(np.trunc(np.float64(w) * (1 << 20))) for w in spec.weights], np.int64)
+    assert np.array_equal(orc.weights_fx(spec.n, spec.nsyn), initial)
 
 
 @pytest.mark.slow
@@ -505,9 +504,9 @@ def test_c2_full_network_prefix(nemo_mod, oracle_mod):
     """The 200k x 1000 network (2e8 synapses) for its first 100 cycles: full
     raster bit-exact against the oracle."""
     spec = examples.izhikevich_80_20(200000, 1000, seed=1)
-    sim = _hip_sim(nemo_mod, spec)
+    sim = hip_sim(nemo_mod, spec)
     orc = oracle_mod.from_spec(spec)
-    _compare_run(sim, orc, 100)
+    compare_run(sim, orc, 100)
 
 
 @pytest.mark.slow
@@ -519,10 +518,10 @@ def test_c3_stdp_epochs(nemo_mod, oracle_mod):
     spec = examples.izhikevich_80_20(n, 1000, seed=3, stdp=True)
     pre, post = examples.stdp_window()
     stdp = (pre, post, 0.0, 1.0, 0.0, -1.0)
-    sim = _hip_sim(nemo_mod, spec, stdp=stdp)
+    sim = hip_sim(nemo_mod, spec, stdp=stdp)
     orc = oracle_mod.from_spec(spec, stdp=stdp)
     for epoch in range(3):
-        _compare_run(sim, orc, 1000)
+        compare_run(sim, orc, 1000)
         sim.apply_stdp(1.0)
         orc.apply_stdp(1.0)
     ref_w = orc.weights_fx(n, spec.nsyn)

@@ -1,6 +1,7 @@
 """NeMo's closed-form step tests on the HIP backend, at several partition
 sizes (the reference's "different partition size => identical raster"
 pattern, src/test/test.cpp:104-132)."""
+import numpy as np
 import pytest
 
 import scenarios
@@ -85,3 +86,154 @@ def test_no_parameter_neuron_type(nemo_mod):
     for c in range(100):
         forced = [c, 999 - c] if c % 3 == 0 else []
         assert sorted(sim.step(forced)) == sorted(set(forced))
+
+
+@pytest.mark.parametrize("n0,nstep", [(1, 1), (1000000, 1), (1, 4), (1000000, 4)])
+def test_non_contiguous_indices(make, n0, nstep):
+    scenarios.non_contiguous_indices(make, n0, nstep)
+
+
+@pytest.mark.parametrize("contiguous", [True, False])
+@pytest.mark.parametrize("stdp", [False, True])
+def test_no_outgoing(make, oracle_mod, contiguous, stdp):
+    """src/test/networks.cpp:15-78 against the oracle: fired lists of every
+    cycle and, with STDP, every weight after applyStdp"""
+    from backends import OracleBuilder
+    for first, last in ((0, 10), (45, 55), (90, 100)):
+        got = scenarios.no_outgoing(make, 100, first, last, contiguous, stdp)
+        ref = scenarios.no_outgoing(lambda: OracleBuilder(oracle_mod), 100, first, last, contiguous, stdp)
+        assert got == ref, (first, last)
+
+
+@pytest.mark.parametrize("start,count", [(0, 1), (0, 4), (1, 4)])
+def test_simple_network(make, oracle_mod, start, count):
+    """Unary network, network without synapses, one-based indices
+    (src/test/test.cpp:178-201) against the oracle"""
+    from backends import OracleBuilder
+    got = scenarios.simple_network(make, start, count)
+    ref = scenarios.simple_network(lambda: OracleBuilder(oracle_mod), start, count)
+    assert got == ref
+
+
+def test_set_neuron(nemo_mod):
+    """src/test/test.cpp:767-962: the neuron setters and getters of the
+    network and of the simulation; invalid neurons and indices throw; a value
+    set on the simulation is used from the next step on and then evolves
+    (it is written once, not again every step); a membrane potential set
+    before the first step is the one the step starts from. With
+    src/test/test.cpp:736-750: the membrane potential of a new simulation is
+    the initial v."""
+    f32 = np.float32
+    a, b = f32(0.02), f32(0.2)
+    c, d = f32(-65.0) + f32(15.0) * f32(0.25), f32(8.0) - f32(6.0) * f32(0.25)
+    v = f32(-65.0)
+    u = b * v
+    sigma, e = f32(5.0), f32(0.1)
+    Err = nemo_mod.NemoError
+    net = nemo_mod.Network()
+    iz = net.add_neuron_type("Izhikevich")
+    with pytest.raises(Err):
+        net.set_neuron(0, a, b, c, d, sigma, u, v)
+    net.add_neuron(iz, 0, a, b, c - f32(0.1), d, sigma, u, v - f32(1.0))
+    for fn, args in ((net.get_neuron_parameter, (1, 0)), (net.get_neuron_state, (1, 0)),
+                     (net.get_neuron_parameter, (0, 5)), (net.get_neuron_state, (0, 2))):
+        with pytest.raises(Err):
+            fn(*args)
+    net.set_neuron(0, a - e, b - e, c - e, d - e, sigma - e, u - e, v - e)
+    assert [net.get_neuron_parameter(0, k) for k in range(5)] == [x - e for x in (a, b, c, d, sigma)]
+    assert [net.get_neuron_state(0, k) for k in range(2)] == [u - e, v - e]
+    for k, x in enumerate((a, b, c, d, sigma)):
+        net.set_neuron_parameter(0, k, x)
+        assert net.get_neuron_parameter(0, k) == x
+    for k, x in enumerate((u, v)):
+        net.set_neuron_state(0, k, x)
+        assert net.get_neuron_state(0, k) == x
+    for fn, args in ((net.set_neuron_parameter, (1, 0, 0.0)), (net.set_neuron_state, (1, 0, 0.0)),
+                     (net.set_neuron_parameter, (0, 5, 0.0)), (net.set_neuron_state, (0, 2, 0.0))):
+        with pytest.raises(Err):
+            fn(*args)
+    conf = nemo_mod.Configuration()
+
+    sim = nemo_mod.Simulation(net, conf)
+    sim.step()
+    sim.set_neuron_state(0, 0, u - e)
+    assert sim.get_neuron_state(0, 0) == u - e
+    sim.set_neuron_state(0, 1, v - e)
+    assert sim.get_neuron_state(0, 1) == v - e
+    u0, v0 = sim.get_neuron_state(0, 0), sim.get_neuron_state(0, 1)
+    for k, x in enumerate((a, b, c, d, sigma)):
+        sim.set_neuron_parameter(0, k, x - e)
+        assert sim.get_neuron_parameter(0, k) == x - e
+    sim.step()
+    assert [sim.get_neuron_parameter(0, k) for k in range(5)] == [x - e for x in (a, b, c, d, sigma)]
+    assert sim.get_neuron_state(0, 0) != u0
+    assert sim.get_neuron_state(0, 1) != v0
+    for fn, args in ((sim.set_neuron_parameter, (1, 0, 0.0)), (sim.set_neuron_state, (1, 0, 0.0)),
+                     (sim.get_neuron_parameter, (1, 0)), (sim.get_neuron_state, (1, 0)),
+                     (sim.set_neuron_parameter, (0, 5, 0.0)), (sim.set_neuron_state, (0, 2, 0.0)),
+                     (sim.get_neuron_parameter, (0, 5)), (sim.get_neuron_state, (0, 2))):
+        with pytest.raises(Err):
+            fn(*args)
+    del sim
+
+    sim = nemo_mod.Simulation(net, conf)
+    sim.step()
+    v0 = sim.get_membrane_potential(0)
+    del sim
+
+    sim = nemo_mod.Simulation(net, conf)
+    assert sim.get_neuron_state(0, 0) == u
+    assert sim.get_neuron_state(0, 1) == v
+    assert sim.get_membrane_potential(0) == v
+    # c is used only when the neuron fires, which it does not in this step
+    sim.set_neuron(0, a, b, c + f32(1.0), d, sigma, u, v)
+    sim.step()
+    assert sim.get_membrane_potential(0) == v0
+    assert [sim.get_neuron_parameter(0, k) for k in range(5)] == [a, b, c + f32(1.0), d, sigma]
+    del sim
+
+    net0 = nemo_mod.Network()
+    net0.add_neuron(net0.add_neuron_type("Izhikevich"), 0, a, b, c, d, 0.0, u, v)
+    sim0 = nemo_mod.Simulation(net0, conf)
+    sim0.step()
+    sim0.step()
+    sim1 = nemo_mod.Simulation(net0, conf)
+    sim1.step()
+    sim1.set_neuron(0, a, b, c, d, 0.0, u, v)
+    sim1.step()
+    sim1.step()
+    assert sim0.get_membrane_potential(0) == sim1.get_membrane_potential(0)
+    del sim0, sim1
+
+    net1 = nemo_mod.Network()
+    net1.add_neuron(net1.add_neuron_type("Izhikevich"), 0, a, b, c, d, sigma, u, v - f32(1.0))
+    sim = nemo_mod.Simulation(net1, conf)
+    sim.set_neuron(0, a, b, c, d, sigma, u, v)
+    assert sim.get_membrane_potential(0) == v
+    sim.step()
+    assert sim.get_membrane_potential(0) == v0
+
+
+def test_synapse_queries_write_only_and_unconnected(nemo_mod):
+    """src/test/test.cpp:605-646: with write-only synapses every synapse
+    query throws; a neuron without synapses has an empty list, and a
+    neuron that does not exist throws."""
+    from backends import HipBuilder
+    Err = nemo_mod.NemoError
+    b = HipBuilder(nemo_mod)
+    scenarios.create_ring(b, 10, plastic=1)
+    b.conf.set_write_only_synapses()
+    sim = b.simulation()
+    sim.step()
+    with pytest.raises(Err):
+        sim.get_synapses_from(0)
+    with pytest.raises(Err):
+        sim.get_synapse_weight((1 << 32) | 0)
+    b = HipBuilder(nemo_mod)
+    for k in range(4):
+        b.add_neuron(scenarios.IZ, k, 0.1, 0.2, 0.3, 0.4, 0.7, 0.5, 0.6)
+    sim = b.simulation()
+    sim.step()
+    with pytest.raises(Err):
+        sim.get_synapses_from(4)
+    assert list(sim.get_synapses_from(3)) == []

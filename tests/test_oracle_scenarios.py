@@ -83,3 +83,24 @@ def test_stdp_in_flight(make, delay):
 
 def test_poisson_and_input(make):
     scenarios.poisson_and_input(make)
+
+
+@pytest.mark.parametrize("n0,nstep", [(1, 1), (1000000, 1), (1, 4), (1000000, 4)])
+def test_non_contiguous_indices(make, n0, nstep):
+    scenarios.non_contiguous_indices(make, n0, nstep)
+
+
+@pytest.mark.parametrize("contiguous", [True, False])
+@pytest.mark.parametrize("stdp", [False, True])
+def test_no_outgoing(make, contiguous, stdp):
+    for first, last in ((0, 10), (45, 55), (90, 100)):
+        fired, weights = scenarios.no_outgoing(make, 100, first, last, contiguous, stdp)
+        assert len(fired) == (60 if stdp else 20)
+        assert len(weights) == (90 * 100 if stdp else 0)
+        assert all(w != 2.0 for w in weights)
+
+
+@pytest.mark.parametrize("start,count", [(0, 1), (0, 4), (1, 4)])
+def test_simple_network(make, start, count):
+    fired, v = scenarios.simple_network(make, start, count)
+    assert len(fired) == 100 and len(v) == count
