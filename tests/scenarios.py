@@ -386,3 +386,29 @@ def simple_network(make, start, count, cycles=100):
     sim = b.simulation()
     fired = [sim.step() for _ in range(cycles)]
     return fired, [sim.get_membrane_potential(start + k) for k in range(count)]
+
+
+def neuron_type_mixture(make, sz_others, iz_first, sz_ring=1024, cycles=1000):
+    """src/test/test.cpp:366-412: a ring of sz_ring Izhikevich neurons and
+    sz_others unconnected PoissonSource neurons (rate 0.001, indices from
+    sz_ring on), the ring's type added first or second, so that simulation
+    indices and neuron indices differ for one of them. Once started, the
+    ring fires neuron t mod sz_ring at cycle t, and anything else that fires
+    is a Poisson source. The fired lists are returned for a comparison
+    between backends."""
+    b = make()
+    if iz_first:
+        create_ring(b, sz_ring)
+    for n in range(sz_ring, sz_ring + sz_others):
+        b.add_neuron("PoissonSource", n, 0.001)
+    if not iz_first:
+        create_ring(b, sz_ring)
+    sim = b.simulation()
+    out = [sim.step([0])]
+    for ms in range(1, cycles):
+        fired = sorted(sim.step())
+        assert fired and fired[0] == ms % sz_ring, (ms, fired[:4])
+        assert all(x >= sz_ring for x in fired[1:]), (ms, fired[:4])
+        out.append(fired)
+    assert sum(len(f) for f in out) > cycles   # some Poisson firing
+    return out

@@ -480,6 +480,43 @@ def test_high_firing(nemo_mod, oracle_mod, partition_size):
         assert sim.step(everyone) == everyone, t
 
 
+@pytest.mark.parametrize("stdp", [False, True])
+@pytest.mark.parametrize("network", ["random", "torus"])
+def test_mapping_partition_sizes(nemo_mod, oracle_mod, network, stdp):
+    """src/test/test.cpp:104-132, 486-512: the random network (1000 x 1000,
+    delay 1) and the torus (one patch, m = 1000, sigma 16), with and without
+    STDP (applyStdp(1.0) after every 1000 cycles, src/test/utils.cpp:9-46),
+    give the same raster at every partition size. Here every partition size
+    is compared with the oracle over 2 simulated seconds. With STDP the
+    plastic weights may grow to 10, so the sums are bounded only by
+    saturation and the saturating variant runs."""
+    import scenarios
+    if network == "random":
+        spec = examples.izhikevich_80_20(1000, 1000, seed=41, dmax=1, stdp=stdp)
+    else:
+        spec = examples.torus(1, 1000, seed=43, stdp=stdp)
+    cfg = scenarios.reference_stdp_config() if stdp else None
+    orc = oracle_mod.from_spec(spec, stdp=cfg)
+    ref = []
+    for s in range(2):
+        ref.append(orc.run(1000))
+        if stdp:
+            orc.apply_stdp(1.0)
+    assert sum(int(c.sum()) for c, _ in ref) > 0
+    for ps in (None, 512, 256, 128, 64):
+        sim = hip_sim(nemo_mod, spec, stdp=cfg, partition_size=ps)
+        assert sim.info()["saturating"] == int(stdp), sim.info()
+        for s in range(2):
+            sim.run(1000)
+            gc, gi = sim.read_fired()
+            d = first_divergence(ref[s][0], ref[s][1], gc, gi)
+            assert d is None, "partition size %s, second %d: first diverging cycle %d: %s" % ((ps, s) + d)
+            if stdp:
+                sim.apply_stdp(1.0)
+        if stdp:
+            compare_weights(sim, orc, spec, 2000)
+
+
 def test_stdp_all_static(nemo_mod, oracle_mod):
     """src/test/test.cpp:661-673: STDP configured but every synapse static
     (1000 neurons x 1000 synapses, delay 1), four epochs of 1000 cycles each

@@ -237,3 +237,12 @@ def test_synapse_queries_write_only_and_unconnected(nemo_mod):
     with pytest.raises(Err):
         sim.get_synapses_from(4)
     assert list(sim.get_synapses_from(3)) == []
+
+
+@pytest.mark.parametrize("iz_first", [True, False])
+def test_neuron_type_mixture(make, oracle_mod, iz_first):
+    """src/test/test.cpp:366-412, and every cycle's fired list against the
+    oracle's"""
+    from backends import OracleBuilder
+    got = scenarios.neuron_type_mixture(make, 1024, iz_first)
+    assert got == scenarios.neuron_type_mixture(lambda: OracleBuilder(oracle_mod), 1024, iz_first)

@@ -104,3 +104,8 @@ def test_no_outgoing(make, contiguous, stdp):
 def test_simple_network(make, start, count):
     fired, v = scenarios.simple_network(make, start, count)
     assert len(fired) == 100 and len(v) == count
+
+
+@pytest.mark.parametrize("iz_first", [True, False])
+def test_neuron_type_mixture(make, iz_first):
+    scenarios.neuron_type_mixture(make, 1024, iz_first)
