@@ -10,6 +10,9 @@ class OracleBuilder(object):
     def __init__(self, oracle_mod):
         self.o = oracle_mod.Oracle()
 
+    def add_type(self, model):
+        """Types need no registration in the oracle"""
+
     def add_neuron(self, model, idx, *args):
         self.o.add_neuron(model, idx, *args)
 
@@ -38,10 +41,13 @@ class HipBuilder(object):
         self.conf = nemo_mod.Configuration()
         self.types = {}
 
-    def add_neuron(self, model, idx, *args):
+    def add_type(self, model):
         if model not in self.types:
             self.types[model] = self.net.add_neuron_type(model)
-        self.net.add_neuron(self.types[model], idx, *args)
+        return self.types[model]
+
+    def add_neuron(self, model, idx, *args):
+        self.net.add_neuron(self.add_type(model), idx, *args)
 
     def add_synapse(self, s, t, d, w, plastic=0):
         return self.net.add_synapse(s, t, d, w, plastic)

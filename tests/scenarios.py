@@ -399,6 +399,7 @@ def neuron_type_mixture(make, sz_others, iz_first, sz_ring=1024, cycles=1000):
     b = make()
     if iz_first:
         create_ring(b, sz_ring)
+    b.add_type("PoissonSource")    # with sz_others = 0: a type with no neurons
     for n in range(sz_ring, sz_ring + sz_others):
         b.add_neuron("PoissonSource", n, 0.001)
     if not iz_first:
@@ -410,5 +411,59 @@ def neuron_type_mixture(make, sz_others, iz_first, sz_ring=1024, cycles=1000):
         assert fired and fired[0] == ms % sz_ring, (ms, fired[:4])
         assert all(x >= sz_ring for x in fired[1:]), (ms, fired[:4])
         out.append(fired)
-    assert sum(len(f) for f in out) > cycles   # some Poisson firing
+    if sz_others:
+        assert sum(len(f) for f in out) > cycles   # some Poisson firing
     return out
+
+
+def poisson_rate(make, duration, other_neurons):
+    """src/test/PoissonSource.cpp:6-30: one PoissonSource of rate 0.01
+    (index 0), optionally after a silent 1024-neuron ring from index 1, fires
+    at least duration * rate times and fewer than three times that. The
+    fired lists are returned for a comparison between backends."""
+    b = make()
+    if other_neurons:
+        create_ring(b, 1024, n0=1)
+    b.add_neuron("PoissonSource", 0, 0.01)
+    sim = b.simulation()
+    out = [sim.step() for _ in range(duration)]
+    nfired = sum(len(f) for f in out)
+    expected = int(abs(np.float32(duration) * np.float32(0.01)))
+    assert expected <= nfired < 3 * expected, (nfired, expected)
+    return out
+
+
+def input_neurons(make, ncount, izhikevich, cycles=1000, seed=3):
+    """src/test/Input.cpp:6-65: ncount Input neurons (no parameters, no
+    state), optionally followed by ncount Izhikevich neurons, fire exactly
+    the neurons forced in each cycle (a random fraction p ~ U[0, 0.5) of
+    them)."""
+    b = make()
+    b.add_type("Input")
+    for i in range(ncount):
+        b.add_neuron("Input", i)
+    if izhikevich:
+        for i in range(ncount, 2 * ncount):
+            b.add_neuron(IZ, i, 0.02, 0.2, -65.0, 8.0, 0.0, np.float32(0.2) * np.float32(-65.0), -65.0)
+    sim = b.simulation()
+    rng = np.random.default_rng(seed)
+    for t in range(cycles):
+        p = 0.5 * rng.random()
+        fstim = [n for n in range(ncount) if rng.random() < p]
+        assert sim.step(fstim) == fstim, t
+
+
+def if_curr_exp_create(make, duration, multiple):
+    """src/test/IF_curr_exp.cpp:16-42: one IF_curr_exp neuron with the
+    arguments NeMo's test passes (in PyNN order, so here v_reset=1, c_m=20,
+    tau_m=5, ...: the header's order is the one the model reads), optionally
+    with a silent 1024-neuron ring from index 1, steps duration cycles. The
+    fired lists and the final state are returned for a comparison between
+    backends."""
+    b = make()
+    b.add_neuron(IF, 0, -65.0, 1.0, 20.0, 5.0, 2.0, 5.0, 0.1, -70.0, -51.0, -65.0, 0.0, 0.0, 1000.0)
+    if multiple:
+        create_ring(b, 1024, n0=1)
+    sim = b.simulation()
+    out = [sim.step() for _ in range(duration)]
+    return out, [sim.get_neuron_state(0, k) for k in range(4)]

@@ -246,3 +246,34 @@ def test_neuron_type_mixture(make, oracle_mod, iz_first):
     from backends import OracleBuilder
     got = scenarios.neuron_type_mixture(make, 1024, iz_first)
     assert got == scenarios.neuron_type_mixture(lambda: OracleBuilder(oracle_mod), 1024, iz_first)
+
+
+def _both(oracle_mod, make, fn, *args):
+    from backends import OracleBuilder
+    got = fn(make, *args)
+    assert got == fn(lambda: OracleBuilder(oracle_mod), *args)
+
+
+@pytest.mark.parametrize("iz_first", [True, False])
+def test_neuron_type_mixture_unused_type(make, oracle_mod, iz_first):
+    """src/test/test.cpp:1041-1043: a neuron type added and left without
+    neurons has no effect"""
+    _both(oracle_mod, make, scenarios.neuron_type_mixture, 0, iz_first)
+
+
+@pytest.mark.parametrize("duration,other", [(1000, False), (10000, False), (1000, True), (10000, True)])
+def test_poisson_rate(make, oracle_mod, duration, other):
+    """src/test/PoissonSource.cpp:6-41, every cycle against the oracle"""
+    _both(oracle_mod, make, scenarios.poisson_rate, duration, other)
+
+
+@pytest.mark.parametrize("ncount,iz", [(1, False), (1000, False), (1, True), (1000, True)])
+def test_input_neurons(make, ncount, iz):
+    """src/test/Input.cpp:6-77"""
+    scenarios.input_neurons(make, ncount, iz)
+
+
+@pytest.mark.parametrize("multiple", [False, True])
+def test_if_curr_exp_create(make, oracle_mod, multiple):
+    """src/test/IF_curr_exp.cpp:16-49, raster and state against the oracle"""
+    _both(oracle_mod, make, scenarios.if_curr_exp_create, 1000, multiple)

@@ -109,3 +109,24 @@ def test_simple_network(make, start, count):
 @pytest.mark.parametrize("iz_first", [True, False])
 def test_neuron_type_mixture(make, iz_first):
     scenarios.neuron_type_mixture(make, 1024, iz_first)
+
+
+@pytest.mark.parametrize("iz_first", [True, False])
+def test_neuron_type_mixture_unused_type(make, iz_first):
+    scenarios.neuron_type_mixture(make, 0, iz_first)
+
+
+@pytest.mark.parametrize("duration,other", [(1000, False), (10000, False), (1000, True), (10000, True)])
+def test_poisson_rate(make, duration, other):
+    scenarios.poisson_rate(make, duration, other)
+
+
+@pytest.mark.parametrize("ncount,iz", [(1, False), (1000, False), (1, True), (1000, True)])
+def test_input_neurons(make, ncount, iz):
+    scenarios.input_neurons(make, ncount, iz)
+
+
+@pytest.mark.parametrize("multiple", [False, True])
+def test_if_curr_exp_create(make, multiple):
+    fired, state = scenarios.if_curr_exp_create(make, 1000, multiple)
+    assert len(fired) == 1000
