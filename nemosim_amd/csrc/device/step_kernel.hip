@@ -303,9 +303,16 @@ __device__ inline void deliver_pass(uint32_t a0, uint32_t b0, uint32_t a1, uint3
 			for(uint32_t i = 0; i < DELIVER_KPL; ++i) {
 				const uint32_t q = ak + gl + i * G;
 				const bool valid = live && (q < bk);
-				/* unconditional load (record 0 stands in for absent ones) so
-				 * no load is left pending on a skipped path */
-				rec[u][i] = ld_rec(syn, valid ? q : 0u);
+				/* the first load of a round is issued by every lane (record 0
+				 * stands in for absent ones); the later ones only when some
+				 * lane of the wave has a record there: buckets rarely reach
+				 * past G records, and the vector memory unit's time goes by
+				 * load instructions, not by the records they carry */
+				if(i == 0 || __any(valid)) {   /* wave-uniform */
+					rec[u][i] = ld_rec(syn, valid ? q : 0u);
+				} else {
+					rec[u][i] = make_uint2(0u, 0u);
+				}
 				ok |= uint32_t(valid) << (u * DELIVER_KPL + i);
 			}
 		}
