@@ -2,6 +2,8 @@
 cycle (first diverging cycle reported), neuron state bit-exact (the FP
 arithmetic keeps NeMo's op order; non-finite values compare by class), and
 synapse weights bit-exact under STDP."""
+import os
+
 import numpy as np
 import pytest
 
@@ -570,3 +572,42 @@ def test_c3_stdp_epochs(nemo_mod, oracle_mod):
         exp = np.float32(np.float32(ref_w[k]) / np.float32(1 << 20))
         assert got == exp, (k, got, exp)
 
+
+
+LONG_PARITY = os.environ.get("NEMOSIM_LONG_PARITY") == "1"
+LONG_REASON = "minutes of oracle time on the host: set NEMOSIM_LONG_PARITY=1 to run"
+
+
+@pytest.mark.slow
+@pytest.mark.skipif(not LONG_PARITY, reason=LONG_REASON)
+def test_c2_10000_cycles(nemo_mod, oracle_mod):
+    """The 200k x 1000 network for 10000 cycles (1.5e10 synaptic events):
+    the full raster bit-exact against the oracle, 1000 cycles at a time,
+    then a sample of the neuron state."""
+    spec = examples.izhikevich_80_20(200000, 1000, seed=1)
+    sim = hip_sim(nemo_mod, spec)
+    orc = oracle_mod.from_spec(spec)
+    for block in range(10):
+        rc, _ = compare_run(sim, orc, 1000)
+        print("C2 cycles %d-%d identical, %d spikes" % (1000 * block, 1000 * block + 999, int(rc.sum())), flush=True)
+    compare_state(sim, orc, spec.n, 2, sample=range(0, spec.n, 97))
+
+
+@pytest.mark.slow
+@pytest.mark.skipif(not LONG_PARITY, reason=LONG_REASON)
+def test_c3_60_epochs(nemo_mod, oracle_mod):
+    """The 50k x 1000 STDP network for 60 simulated seconds with
+    applyStdp(1.0) after each: every second's raster bit-exact against the
+    oracle, then a sample of 200000 weights."""
+    n = 50000
+    spec = examples.izhikevich_80_20(n, 1000, seed=3, stdp=True)
+    pre, post = examples.stdp_window()
+    stdp = (pre, post, 0.0, 1.0, 0.0, -1.0)
+    sim = hip_sim(nemo_mod, spec, stdp=stdp)
+    orc = oracle_mod.from_spec(spec, stdp=stdp)
+    for epoch in range(60):
+        rc, _ = compare_run(sim, orc, 1000)
+        sim.apply_stdp(1.0)
+        orc.apply_stdp(1.0)
+        print("C3 second %d identical, %d spikes" % (epoch, int(rc.sum())), flush=True)
+    compare_weights(sim, orc, spec, 200000, seed=5)
