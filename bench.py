@@ -277,8 +277,10 @@ def bench_serve(args):
     from nemosim_amd import serving
     reqs = serving.poisson_requests(args.serve_requests, rate_hz=50.0, seed=args.seed)
     budget = int(args.serve_budget_gb * (1 << 30))
+    from nemosim_amd import autotune
     srv = serving.Server(hbm_budget_bytes=budget, chunk_cycles=args.serve_chunk,
-                         build_workers=args.serve_workers, keep_rasters=False)
+                         build_workers=args.serve_workers, keep_rasters=False,
+                         tuning=None if args.serve_untuned else autotune.load_table())
     log("serve: %d requests, budget %.0f GiB, chunks of %d cycles" % (len(reqs), args.serve_budget_gb, args.serve_chunk))
     stats = srv.serve(reqs, realtime=True)
     return {
@@ -338,6 +340,8 @@ def main():
     ap.add_argument("--serve-budget-gb", type=float, default=64.0)
     ap.add_argument("--serve-chunk", type=int, default=100)
     ap.add_argument("--serve-workers", type=int, default=8)
+    ap.add_argument("--serve-untuned", action="store_true", help="serving: backend default partition sizes, "
+                    "not the tuning table")
     args = ap.parse_args()
 
     if args.gpus != 1:

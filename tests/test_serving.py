@@ -24,8 +24,12 @@ def test_budget_admission_order_cpu():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("device_construction", [False, True])
-def test_alone_equals_mix(oracle_mod, device_construction):
+@pytest.mark.parametrize("device_construction,tuned", [(False, False), (True, False), (False, True)])
+def test_alone_equals_mix(oracle_mod, device_construction, tuned):
+    """With tuned set the server takes partition sizes and delivery lanes
+    from the tuning table while the requests run alone with the backend's
+    defaults: the rasters must still be identical."""
+    from nemosim_amd import autotune
     reqs = [
         serving.Request(0, 0.00, 1500, 100, 350, seed=31),
         serving.Request(1, 0.00, 4000, 80, 250, model="IF_curr_exp_noise", seed=32),
@@ -37,7 +41,8 @@ def test_alone_equals_mix(oracle_mod, device_construction):
     budget = (reqs[0].estimated_device_bytes() + reqs[1].estimated_device_bytes() + reqs[2].estimated_device_bytes()
               + max(r.construction_bytes() for r in reqs))
     srv = serving.Server(hbm_budget_bytes=budget, chunk_cycles=64, build_workers=2,
-                         device_construction=device_construction)
+                         device_construction=device_construction,
+                         tuning=autotune.load_table() if tuned else None)
     stats = srv.serve(reqs, realtime=False)
     assert stats["requests"] == 4
     assert stats["p99_latency_s"] >= stats["p50_latency_s"] > 0
