@@ -280,7 +280,7 @@ def bench_serve(args):
     from nemosim_amd import autotune
     srv = serving.Server(hbm_budget_bytes=budget, chunk_cycles=args.serve_chunk,
                          build_workers=args.serve_workers, keep_rasters=False,
-                         tuning=None if args.serve_untuned else autotune.load_table())
+                         tuning=autotune.load_table() if args.serve_tuned else None)
     log("serve: %d requests, budget %.0f GiB, chunks of %d cycles" % (len(reqs), args.serve_budget_gb, args.serve_chunk))
     stats = srv.serve(reqs, realtime=True)
     return {
@@ -340,8 +340,8 @@ def main():
     ap.add_argument("--serve-budget-gb", type=float, default=64.0)
     ap.add_argument("--serve-chunk", type=int, default=100)
     ap.add_argument("--serve-workers", type=int, default=8)
-    ap.add_argument("--serve-untuned", action="store_true", help="serving: backend default partition sizes, "
-                    "not the tuning table")
+    ap.add_argument("--serve-tuned", action="store_true", help="serving: partition sizes and delivery lanes "
+                    "from the batch tuning table instead of the backend's defaults")
     args = ap.parse_args()
 
     if args.gpus != 1:
