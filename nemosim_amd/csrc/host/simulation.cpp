@@ -292,17 +292,22 @@ Simulation::Simulation(const NetworkImpl& net, const ConfigurationImpl& conf) :
 	}
 
 	/* T: neurons per partition. The automatic choice takes at most one
-	 * partition per compute unit (256), and for mid-size networks at most
-	 * about 40 partitions of up to 512 neurons: their cycle is bound by the
-	 * exchange between partitions, and fewer, wider partitions measured
-	 * faster (10k x 100: 7.2 -> 6.3 us per cycle; 50k x 1000: 10.4 -> 9.9 us).
-	 * The LDS ring [D][T] bounds it from above. */
+	 * partition per compute unit (256), and for mid-size networks fewer,
+	 * wider partitions: their cycle is bound by the exchange between
+	 * partitions. Measured per cycle: about 40 partitions of up to 512
+	 * neurons (10k x 100: 7.2 -> 6.3 us; 50k x 1000: 10.4 -> 9.7 us), and
+	 * with rows of 500 synapses or more, from 1.5k to 12k neurons, about 13
+	 * partitions (2k x 1000: 8.0 -> 7.2 us; 5k x 1000: 9.6 -> 7.1 us; 10k x
+	 * 1000: 10.2 -> 8.6 us). The LDS ring [D][T] bounds it from above. */
 	uint32_t T;
 	if(conf.partitionSize) {
 		T = std::min(1024u, roundUp64(conf.partitionSize));
 	} else {
 		const uint32_t perCU = roundUp64(std::max<uint32_t>(1u, (m_n + 255u) / 256u));
-		const uint32_t wide = std::min(512u, roundUp64(std::max<uint32_t>(1u, (m_n + 39u) / 40u)));
+		const bool longRows = m_n >= 1500u && m_n <= 12000u && m_nsyn / m_n >= 500u;
+		const uint32_t parts = longRows ? 13u : 40u;
+		const uint32_t wide = std::min(longRows ? 1024u : 512u,
+				roundUp64(std::max<uint32_t>(1u, (m_n + parts - 1u) / parts)));
 		T = std::max(64u, std::min(1024u, std::max(perCU, wide)));
 	}
 	auto ldsFor = [&](uint32_t t) {
