@@ -152,6 +152,10 @@ static int run(const char* netfile, const char* rasterfile, int use_fstim, int u
 	float* args = malloc(sizeof(float) * 7u * ncount);
 	uint32_t* tgt = malloc(sizeof(uint32_t) * ncount * scount);
 	float* wgt = malloc(sizeof(float) * ncount * scount);
+	if(args == NULL || tgt == NULL || wgt == NULL) {
+		fprintf(stderr, "out of host memory\n");
+		return 1;
+	}
 
 	nemo_network_t net = nemo_new_network();
 	unsigned iz;
