@@ -118,8 +118,10 @@ runBatch(Simulation* const sims[], size_t n, const unsigned nsteps[], unsigned l
 	std::lock_guard<std::mutex> lock(g_planMutex);
 	std::lock_guard<std::recursive_mutex> devLock(deviceMutex());
 	/* Plugin-path simulations and empty work run on their own; the rest are
-	 * grouped by (device, kernel variant), keeping the given order within a
-	 * group, and packed greedily into launches. */
+	 * grouped by (device, kernel variant) and, within a group, ordered by
+	 * workgroup size and LDS bytes, then packed greedily into launches. A
+	 * launch runs every workgroup at its largest member's size and LDS, so
+	 * networks of like partitions share a launch and more of them fit. */
 	std::vector<size_t> order;
 	for(size_t i = 0; i < n; ++i) {
 		Simulation* sim = sims[i];
@@ -133,8 +135,12 @@ runBatch(Simulation* const sims[], size_t n, const unsigned nsteps[], unsigned l
 		}
 	}
 	std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) {
-		const int dx = sims[x]->device(), dy = sims[y]->device();
-		return dx != dy ? dx < dy : sims[x]->variant() < sims[y]->variant();
+		const Simulation* a = sims[x];
+		const Simulation* b = sims[y];
+		if(a->device() != b->device()) return a->device() < b->device();
+		if(a->variant() != b->variant()) return a->variant() < b->variant();
+		if(a->blockThreads() != b->blockThreads()) return a->blockThreads() < b->blockThreads();
+		return a->ldsBytes() < b->ldsBytes();
 	});
 	size_t i = 0;
 	while(i < order.size()) {
