@@ -118,10 +118,12 @@ runBatch(Simulation* const sims[], size_t n, const unsigned nsteps[], unsigned l
 	std::lock_guard<std::mutex> lock(g_planMutex);
 	std::lock_guard<std::recursive_mutex> devLock(deviceMutex());
 	/* Plugin-path simulations and empty work run on their own; the rest are
-	 * grouped by (device, kernel variant) and, within a group, ordered by
-	 * workgroup size and LDS bytes, then packed greedily into launches. A
-	 * launch runs every workgroup at its largest member's size and LDS, so
-	 * networks of like partitions share a launch and more of them fit. */
+	 * grouped by (device, kernel variant) and, within a group, by workgroup
+	 * size, keeping the given order within each size, then packed greedily
+	 * into launches. A launch runs every workgroup at its largest member's
+	 * size and LDS, so networks of like partitions share a launch and more
+	 * of them fit; a batch that changes by a few networks between calls
+	 * (serving) keeps most of its launches as they were. */
 	std::vector<size_t> order;
 	for(size_t i = 0; i < n; ++i) {
 		Simulation* sim = sims[i];
@@ -139,8 +141,7 @@ runBatch(Simulation* const sims[], size_t n, const unsigned nsteps[], unsigned l
 		const Simulation* b = sims[y];
 		if(a->device() != b->device()) return a->device() < b->device();
 		if(a->variant() != b->variant()) return a->variant() < b->variant();
-		if(a->blockThreads() != b->blockThreads()) return a->blockThreads() < b->blockThreads();
-		return a->ldsBytes() < b->ldsBytes();
+		return a->blockThreads() < b->blockThreads();
 	});
 	size_t i = 0;
 	while(i < order.size()) {
