@@ -19,9 +19,16 @@
  *   - step semantics: the reference's own closed-form tests (ring, double
  *     ring, fstim, istim, saturation, STDP closed form) and brute-force
  *     current sums on small random networks.
- *   - IF_curr_exp_noise is a build-defined model (no reference counterpart);
- *     its noise term follows the Izhikevich convention.  Parity unpinned
- *     beyond the IF_curr_exp closed-form checks.
+ *   - neuron updates (update_izhikevich, update_if_curr_exp with and without
+ *     noise): replayed independently in float64/float32 from the
+ *     reference's plugin sources, cycle by cycle over thousands of neurons
+ *     with synaptic, external and Gaussian input (test_oracle_model_pins);
+ *     each plausible mistake (float32 right-hand side, no early exit, fused
+ *     multiply-add, noise order, threshold and refractory comparisons) is
+ *     shown to change the results on the same inputs.
+ *   - IF_curr_exp_noise is a build-defined model (no reference counterpart):
+ *     IF_curr_exp plus sigma * nrand added once the total current is
+ *     formed, pinned against that definition as above.
  *
  * One deliberate deviation: a spike at delay 64 (bit 63 of the firing
  * history) would make the reference's delivery loop shift a 64-bit word by
