@@ -59,12 +59,57 @@ def algorithmic_bytes(fired_ids, deg, rows, n_neurons, steps):
     return 8 * e + 8 * r + 68 * n_neurons * steps, e, r
 
 
-def cpu_baseline(spec, target_s, warm_steps, stdp=None):
+def oracle_leg(spec, run, target_s, parity_s, warm_steps=200, stdp=None, baseline=True):
+    """The oracle (a restatement of NeMo's CPU backend) on the same network:
+
+    parity: replays the GPU run's cycles from cycle 0 -- in the same launch
+    chunks, with applyStdp after the same ones -- for as long as parity_s of
+    host time allows, and compares every cycle's fired set with the GPU's;
+    cpu_baseline: then times a bounded sample of further cycles (about
+    target_s seconds) on this host's cores.
+
+    Returns (parity, cpu_baseline); cpu_baseline is None unless baseline."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
-    log("cpu baseline: building the oracle network ...")
+    log("oracle: building the network ...")
     orc = oracle.from_spec(spec, stdp=stdp)
-    orc.run(warm_steps)
+    counts, ids, chunks = run["counts"], run["ids"], run["chunks"]
+    offs = np.concatenate([[0], np.cumsum(counts.astype(np.int64))])
+    t_start = time.perf_counter()
+    checked, cycle, diverged = 0, 0, None
+    for k, apply in chunks:
+        if time.perf_counter() - t_start > parity_s:
+            break
+        rc, ri = orc.run(k)
+        gc = counts[cycle:cycle + k]
+        gi = ids[offs[cycle]:offs[cycle + k]]
+        d = oracle.first_divergence(rc, ri, gc, gi)
+        if d is not None:
+            diverged = (cycle + d[0], d[1])
+            break
+        if apply:
+            orc.apply_stdp(1.0)
+        cycle += k
+        checked = cycle
+    timed0, timed1 = run["timed"]
+    if diverged is not None:
+        status = "DIVERGED at cycle %d (%s)" % diverged
+    elif checked >= timed1:
+        status = "bit-exact: every cycle 0..%d, all timed cycles %d..%d included" % (checked - 1, timed0, timed1 - 1)
+    elif checked > timed0:
+        status = "bit-exact: cycles 0..%d (timed cycles %d..%d of %d..%d)" % (checked - 1, timed0, checked - 1,
+                                                                              timed0, timed1 - 1)
+    else:
+        status = "bit-exact: cycles 0..%d (warm-up only; no timed cycle checked)" % (checked - 1)
+    parity = {"status": status, "checked_cycles": [0, checked], "timed_cycles": [timed0, timed1],
+              "first_divergence": None if diverged is None else diverged[0],
+              "oracle_s": time.perf_counter() - t_start}
+    log("parity: " + status)
+    if not baseline or diverged is not None:
+        return parity, None
+    if cycle < warm_steps:
+        orc.run(warm_steps - cycle)
+        cycle = warm_steps
     deg, _ = delay_tables(spec)
     t0 = time.perf_counter()
     orc.run(10)
@@ -72,27 +117,29 @@ def cpu_baseline(spec, target_s, warm_steps, stdp=None):
     nsamp = int(max(20, min(20000, target_s / max(probe, 1e-6))))
     log("cpu baseline: %.2f ms/cycle probe; sampling %d cycles" % (probe * 1e3, nsamp))
     t0 = time.perf_counter()
-    counts, ids = orc.run(nsamp)
+    _, sids = orc.run(nsamp)
     wall = time.perf_counter() - t0
-    events = int(deg[ids].sum())
-    return {
+    events = int(deg[sids].sum())
+    first = cycle + 10
+    return parity, {
         "value": events / wall,
         "unit": "synaptic events/s",
         "cores": int(oracle.load().oracle_num_threads()),
         "kind": "port",
-        "sample": "same %d-neuron network, cycles %d..%d after %d warm-up cycles (%.1f s, %d events); "
-                  "OpenMP restatement of NeMo's CPU backend" % (spec.n, warm_steps + 10, warm_steps + 10 + nsamp,
-                                                              warm_steps, wall, events),
+        "sample": "same %d-neuron network, cycles %d..%d (%.1f s, %d events); "
+                  "OpenMP restatement of NeMo's CPU backend" % (spec.n, first, first + nsamp - 1, wall, events),
         "ms_per_sim_second": wall * 1e3 / (nsamp / 1000.0),
     }
 
 
-def build_sim(nemosim_amd, spec, stdp=None, partition_size=None, mode="auto", lanes=0):
+def build_sim(nemosim_amd, spec, stdp=None, partition_size=None, mode="auto", lanes=0, exchange_waves=0):
     net = spec.to_nemo(nemosim_amd)
     conf = nemosim_amd.Configuration()
     conf.set_cuda_backend(0)
     conf.set_write_only_synapses()
     conf.set_step_mode(mode)
+    if exchange_waves:
+        conf.set_exchange_waves(exchange_waves)
     if partition_size:
         conf.set_partition_size(partition_size)
     if lanes:
@@ -103,33 +150,52 @@ def build_sim(nemosim_amd, spec, stdp=None, partition_size=None, mode="auto", la
     return nemosim_amd.Simulation(net, conf)
 
 
-def timed_runs(sim, spec, steps, warmup, L, stdp_period=None):
-    """warm-up, then `steps` timed cycles in launches of L; returns
-    (wall_s, kernel_ms, events, fired ids, launches)."""
+def timed_runs(sim, steps, warmup, L, stdp_period=None):
+    """warm-up, then `steps` timed cycles in launches of L. The raster of
+    every launch, warm-up included, is read back outside the timed interval.
+    Returns wall_s, kernel_ms, events, launches and the run record for the
+    parity check: per-cycle fired counts and ids, the launch chunks
+    (cycles, applyStdp after it) and the timed cycle range."""
+    counts, ids, chunks = [], [], []
+
+    def launch(k, done_after):
+        ev = sim.run(k)
+        apply = bool(stdp_period) and done_after % stdp_period == 0
+        if apply:
+            sim.apply_stdp(1.0)
+        return ev, apply
+
     done = 0
     while done < warmup:
         k = min(L, warmup - done)
-        sim.run(k)
+        _, apply = launch(k, done + k)
+        c, i = sim.read_fired()
+        counts.append(c)
+        ids.append(i)
+        chunks.append((k, apply))
         done += k
-        if stdp_period and done % stdp_period == 0:
-            sim.apply_stdp(1.0)
     events, kernel_ms, wall, launches = 0, 0.0, 0.0, 0
-    fired = []
     done = 0
     while done < steps:
         k = min(L, steps - done)
         t0 = time.perf_counter()
-        events += sim.run(k)
-        if stdp_period and (warmup + done + k) % stdp_period == 0:
-            sim.apply_stdp(1.0)
+        ev, apply = launch(k, warmup + done + k)
         wall += time.perf_counter() - t0
+        events += ev
         kernel_ms += sim.info()["last_kernel_ms"]
         launches += 1
-        _, ids = sim.read_fired()      # outside the timed interval
-        fired.append(ids)
+        c, i = sim.read_fired()        # outside the timed interval
+        counts.append(c)
+        ids.append(i)
+        chunks.append((k, apply))
         done += k
-    fired = np.concatenate(fired) if fired else np.zeros(0, np.uint32)
-    return wall, kernel_ms, events, fired, launches
+    run = {
+        "counts": np.concatenate(counts).astype(np.uint32) if counts else np.zeros(0, np.uint32),
+        "ids": np.concatenate(ids).astype(np.uint32) if ids else np.zeros(0, np.uint32),
+        "chunks": chunks,
+        "timed": (warmup, warmup + steps),
+    }
+    return wall, kernel_ms, events, launches, run
 
 
 def bench_plain(args, stdp=False):
@@ -146,14 +212,17 @@ def bench_plain(args, stdp=False):
         pre, post = examples.stdp_window()
         stdp_fn = (pre, post, 0.0, 1.0, 0.0, -1.0)
     t0 = time.perf_counter()
-    sim = build_sim(nemosim_amd, spec, stdp=stdp_fn, partition_size=args.partition_size or None)
+    sim = build_sim(nemosim_amd, spec, stdp=stdp_fn, partition_size=args.partition_size or None,
+                    exchange_waves=args.exchange_waves)
     info = sim.info()
     log("  simulation built in %.1f s: %s" % (time.perf_counter() - t0, info))
     L = max(1, min(args.launch_steps, args.steps))
     deg, rows = delay_tables(spec)
     log("warm-up %d cycles, timed %d cycles in launches of %d" % (args.warmup, args.steps, L))
-    wall, kernel_ms, events, fired, launches = timed_runs(sim, spec, args.steps, args.warmup, L,
-                                                          stdp_period=1000 if stdp else None)
+    wall, kernel_ms, events, launches, run = timed_runs(sim, args.steps, args.warmup, L,
+                                                        stdp_period=1000 if stdp else None)
+    t0c = args.warmup
+    fired = run["ids"][int(run["counts"][:t0c].sum()):]       # the timed cycles' spikes
     abytes, e_check, r_total = algorithmic_bytes(fired.astype(np.int64), deg, rows, n, args.steps)
     if e_check != events:
         log("warning: device event count %d != raster-derived %d" % (events, e_check))
@@ -202,9 +271,13 @@ def bench_plain(args, stdp=False):
             "kernel_ms_per_launch": kernel_ms / launches,
         },
         "cpu_baseline": None,
+        "parity": None,
     }
-    if not args.no_cpu_baseline:
-        result["cpu_baseline"] = cpu_baseline(spec, args.cpu_seconds, warm_steps=200, stdp=stdp_fn)
+    if not args.no_parity or not args.no_cpu_baseline:
+        parity, base = oracle_leg(spec, run, args.cpu_seconds, args.parity_seconds, stdp=stdp_fn,
+                                  baseline=not args.no_cpu_baseline)
+        result["parity"] = parity
+        result["cpu_baseline"] = base
     return result
 
 
@@ -330,9 +403,14 @@ def main():
     ap.add_argument("--neurons", type=int, default=200000)
     ap.add_argument("--synapses", type=int, default=1000)
     ap.add_argument("--partition-size", type=int, default=0, help="neurons per workgroup (0: automatic)")
+    ap.add_argument("--exchange-waves", type=int, default=0,
+                    help="waves per workgroup running the exchange (0: none)")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-parity", action="store_true", help="skip the oracle replay of the GPU run's cycles")
+    ap.add_argument("--parity-seconds", type=float, default=60.0,
+                    help="host time the oracle may spend replaying the GPU run's cycles")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "nemo_step_kernel_c2_counters.txt"))
     ap.add_argument("--batch-networks", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=2000)

@@ -68,6 +68,15 @@ nemo_status_t nemo_set_stdp_lds_staging(nemo_configuration_t conf, int enable);
  * host (src/nemo/ConnectivityMatrix.cpp:98-161). */
 nemo_status_t nemo_set_device_construction(nemo_configuration_t conf, int enable);
 
+/* Waves of each step-kernel workgroup that run the exchange between
+ * partitions and the delivery of delay-1 records, while the others deliver
+ * the previous cycle's records of longer delays: n > 0 = n waves (at least
+ * one other must remain; needs deferred delivery and LDS room for the far
+ * list), 0 (default) or < 0 = none (every wave does both, one after the
+ * other). Same results either way. No counterpart in NeMo; a tuning knob of
+ * the fused step kernel. */
+nemo_status_t nemo_set_exchange_waves(nemo_configuration_t conf, int waves);
+
 /* Which step implementation runs a simulation:
  *   NEMO_STEP_AUTO    the fused persistent kernel when every neuron type is a
  *                     built-in model and the network fits it; else plugins
@@ -77,6 +86,26 @@ nemo_status_t nemo_set_device_construction(nemo_configuration_t conf, int enable
  * Both give identical results. */
 enum { NEMO_STEP_AUTO = 0, NEMO_STEP_FUSED = 1, NEMO_STEP_PLUGINS = 2 };
 nemo_status_t nemo_set_step_mode(nemo_configuration_t conf, int mode);
+
+/* ------------------------------------------------ accessors for nemo.hpp */
+
+/* Accessors behind the C++ surface (include/nemo.hpp) that the C API of
+ * src/nemo.h lacks. */
+/* A copy of conf (nemo::Configuration's copy constructor,
+ * src/nemo/Configuration.hpp:49); NULL on failure. */
+nemo_configuration_t nemo_copy_configuration(nemo_configuration_t conf);
+/* Configuration::disableLogging / loggingEnabled (src/nemo/Configuration.hpp:58-59) */
+nemo_status_t nemo_disable_logging(nemo_configuration_t conf);
+nemo_status_t nemo_logging_enabled(nemo_configuration_t conf, int* enabled);
+/* Configuration::cudaPartitionSize (src/nemo/Configuration.hpp:62) */
+nemo_status_t nemo_partition_size(nemo_configuration_t conf, unsigned* neurons);
+/* Configuration::writeOnlySynapses (src/nemo/Configuration.hpp:106) */
+nemo_status_t nemo_write_only_synapses(nemo_configuration_t conf, int* enabled);
+/* Network::maxDelay / minWeight / maxWeight (src/nemo/Network.hpp:202-205);
+ * the weights' range starts at 0 (src/nemo/NetworkImpl.cpp:28-30). */
+nemo_status_t nemo_max_delay(nemo_network_t net, unsigned* delay);
+nemo_status_t nemo_min_weight(nemo_network_t net, float* weight);
+nemo_status_t nemo_max_weight(nemo_network_t net, float* weight);
 
 /* ------------------------------------------------------- multi-cycle runs */
 
@@ -147,6 +176,8 @@ typedef struct nemo_hip_info {
 	unsigned stdp_staged_chunks; /* STDP chunks whose reverse records are staged in LDS per cycle */
 	unsigned stdp_prefix_ring;   /* 1: fired-count prefixes of the last postLen cycles kept in LDS */
 	unsigned device_built;       /* 1: synapse structures were built on the device */
+	unsigned exchange_waves;     /* waves per workgroup running the exchange and delay-1 delivery */
+	unsigned far_list_sources;   /* fired sources per cycle whose far records wait in the LDS far list */
 } nemo_hip_info_t;
 
 nemo_status_t nemo_hip_get_info(nemo_simulation_t sim, nemo_hip_info_t* info);

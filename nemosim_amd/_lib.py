@@ -51,6 +51,8 @@ class InfoT(ctypes.Structure):
         ("stdp_staged_chunks", ctypes.c_uint),
         ("stdp_prefix_ring", ctypes.c_uint),
         ("device_built", ctypes.c_uint),
+        ("exchange_waves", ctypes.c_uint),
+        ("far_list_sources", ctypes.c_uint),
     ]
 
 
@@ -122,6 +124,15 @@ SIGNATURES = {
     "nemo_set_partition_size": (ctypes.c_int, [c_void_p, ctypes.c_uint]),
     "nemo_set_delivery_lanes": (ctypes.c_int, [c_void_p, ctypes.c_uint]),
     "nemo_set_stdp_lds_staging": (ctypes.c_int, [c_void_p, ctypes.c_int]),
+    "nemo_set_exchange_waves": (ctypes.c_int, [c_void_p, ctypes.c_int]),
+    "nemo_copy_configuration": (c_void_p, [c_void_p]),
+    "nemo_disable_logging": (ctypes.c_int, [c_void_p]),
+    "nemo_logging_enabled": (ctypes.c_int, [c_void_p, ctypes.POINTER(ctypes.c_int)]),
+    "nemo_partition_size": (ctypes.c_int, [c_void_p, c_uint_p]),
+    "nemo_write_only_synapses": (ctypes.c_int, [c_void_p, ctypes.POINTER(ctypes.c_int)]),
+    "nemo_max_delay": (ctypes.c_int, [c_void_p, c_uint_p]),
+    "nemo_min_weight": (ctypes.c_int, [c_void_p, ctypes.POINTER(ctypes.c_float)]),
+    "nemo_max_weight": (ctypes.c_int, [c_void_p, ctypes.POINTER(ctypes.c_float)]),
     "nemo_set_device_construction": (ctypes.c_int, [c_void_p, ctypes.c_int]),
     "nemo_set_step_mode": (ctypes.c_int, [c_void_p, ctypes.c_int]),
     "nemo_hip_run": (ctypes.c_int, [c_void_p, ctypes.c_uint, ctypes.POINTER(ScheduleT), c_ull_p]),

@@ -52,6 +52,11 @@ typedef struct NetDev {
 	uint32_t glog2;      /* log2 of the lanes that share one fired source in delivery (0..6) */
 	uint32_t flRing;     /* fired lists and counts kept for this many cycles (buffer t mod flRing) */
 	uint32_t stdpCache;  /* STDP chunks whose reverse records are staged in LDS (64 words each) */
+	uint32_t exWaves;    /* with NEMO_NET_DEFER: waves of a workgroup that run the exchange and the
+	                      * delay-1 delivery while the others deliver the previous cycle's far
+	                      * records (0: every wave does both, one after the other) */
+	uint32_t farCap;     /* fired sources per cycle whose far ranges fit the LDS far list (multiple
+	                      * of 64; with exWaves > 0) */
 	uint64_t cycle0;     /* cycle count at the start of the launch */
 
 	const uint8_t* model;     /* [n] model id */
@@ -111,6 +116,10 @@ static inline uint32_t nemo_step_variant(uint32_t flags)
 }
 int nemo_dev_step_launch(const NetDev* nets_dev, const WorkItem* work_dev, uint32_t nblocks,
 		uint32_t blockDim, uint32_t ldsBytes, uint32_t variant, void* stream);
+/* One network: the descriptor is passed by value (nd_host is read at the
+ * call), the work table maps blocks to its partitions. */
+int nemo_dev_step_launch_one(const NetDev* nd_host, const WorkItem* work_dev, uint32_t nblocks,
+		uint32_t blockDim, uint32_t ldsBytes, uint32_t variant, void* stream);
 int nemo_dev_step_max_blocks(uint32_t blockDim, uint32_t ldsBytes, uint32_t variant, uint32_t* maxBlocks);
 /* Batched step as an instantiated HIP graph (clear -> cooperative step ->
  * collect); collect_dev receives [nnets][8] words: the first 8 words of each
@@ -137,7 +146,11 @@ int nemo_dev_nrand_seq(const uint32_t* state4_dev, float* out_dev, size_t n, voi
 /* LDS bytes the step kernel needs for partitions of T neurons, ring depth D,
  * P partitions. Optional STDP areas follow, in this order, when the host
  * finds room for them: stdpCache x 256 bytes of staged reverse records, then
- * with NEMO_NET_PREF_RING postLen x ((P+1) rounded up to 64) x 4 bytes. */
+ * with NEMO_NET_PREF_RING postLen x ((P+1) rounded up to 64) x 4 bytes.
+ * With exchange waves the far list follows: NEMO_FAR_LDS_BYTES(farCap). */
+/* far list: [2][farCap] range starts, [2][farCap] range ends, two counts
+ * and the exchange waves' flag */
+#define NEMO_FAR_LDS_BYTES(cap) (16u * (cap) + 16u)
 
 static inline uint32_t nemo_step_lds_bytes(uint32_t T, uint32_t D, uint32_t P, int saturate, int stdp)
 {

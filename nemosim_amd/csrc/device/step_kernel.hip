@@ -229,7 +229,7 @@ constexpr uint32_t OFF_NEAR_SHIFT = 28u;
 /* rounds of record loads in flight per batch: fewer where the variant has
  * fewer registers left (saturating accumulation; any model, 10 parameters
  * and 4 state variables per neuron) */
-template<bool SAT, bool IZO> struct DeliverRounds { static constexpr uint32_t value = SAT ? 4u : (IZO ? 8u : 6u); };
+template<bool SAT, bool IZO> struct DeliverRounds { static constexpr uint32_t value = SAT ? 4u : (IZO ? 6u : 5u); };
 constexpr uint32_t DELIVER_KPL = 2u;
 
 /* Partition whose fired list holds flat index j: the q with
@@ -476,6 +476,198 @@ __device__ inline void stdp_accumulate(const uint2* work, uint32_t nw, uint32_t*
 	}
 }
 
+/* ---- exchange waves and far waves (networks with deferred delivery).
+ *
+ * After publishing cycle t, the first exWaves waves of a workgroup run the
+ * latency-critical chain of the cycle: wait for every partition's count,
+ * read the fired-list entries, the bucket bounds of this partition (loaded
+ * straight into LDS) and the delay-1 records, whose weights go to slot t+1
+ * before the next update. The rest of every bucket (delays >= 2) is left as
+ * a range in the LDS far list. Meanwhile the other waves deliver the far
+ * list of cycle t-1 into slots t+1 .. t-1+D, which the update of t+1 does
+ * not read beyond slot t+1, and the delay-2 records of cycle t-1, bound for
+ * slot t+1, are done before the cycle's closing barrier. */
+/* fired sources per exchange lane per pass (fewer where the variant has
+ * fewer registers left) */
+template<bool SAT, bool IZO> struct ExSources { static constexpr uint32_t value = (SAT && !IZO) ? 4u : 6u; };
+constexpr uint32_t EX_NEAR = 4u;   /* delay-1 records per lane loaded together */
+
+template<bool SAT, bool IZO>
+__device__ inline unsigned long long exchange_phase(const NetDev& nd, uint32_t p, uint64_t t, uint32_t k,
+		uint32_t slot, uint32_t flb, uint32_t wave, uint32_t lane, uint32_t* pref, uint32_t* exFlag,
+		uint32_t* FA, uint32_t* FB, uint32_t* fcnt, uint32_t* pringDst, const NEMO_GLOBAL uint2* syn,
+		const NEMO_GLOBAL uint32_t* off, uint32_t offMask, int32_t* ringE, int32_t* ringI, uint32_t* ovf,
+		uint32_t fwords)
+{
+	constexpr uint32_t EX_K = ExSources<SAT, IZO>::value;
+	const uint32_t P = nd.P, T = nd.T, D = nd.D, exw = nd.exWaves, cap = nd.farCap;
+	const uint32_t tag = uint32_t(t);
+	/* wave 0 waits for the counts; the other exchange waves for wave 0 */
+	if(wave == 0) {
+		gather_counts(glb(nd.fc) + flb * P, P, tag, pref, lane, nd.spinLimit, nd.status, p);
+		if(lane == 0) {
+			__hip_atomic_store(exFlag, k + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+		}
+	}
+	while(__hip_atomic_load(exFlag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) != k + 1u) {
+		__builtin_amdgcn_s_sleep(1);
+	}
+	const uint32_t F = pref[P];
+	const uint32_t lim = min(F, cap);
+	const uint32_t nEl = exw * 64u;
+	const uint32_t el = wave * 64u + lane;
+	if(el == 0) *fcnt = lim;
+	if(pringDst) {
+		for(uint32_t i = el; i <= P; i += nEl) pringDst[i] = pref[i];
+	}
+	const NEMO_GLOBAL uint64_t* fl = glb(nd.fl) + size_t(flb) * P * T;
+	const uint32_t s1 = (slot + 1u >= D) ? slot + 1u - D : slot + 1u;   /* slot of cycle t+1 */
+	unsigned long long events = 0;
+	for(uint32_t base = 0; base < lim; base += nEl * EX_K) {   /* wave-uniform */
+		/* the fired-list entries: EX_K sources per lane, loads in flight together */
+		uint64_t ev[EX_K];
+		uint32_t eo[EX_K];
+#pragma unroll
+		for(uint32_t i = 0; i < EX_K; ++i) {
+			const uint32_t j = base + i * nEl + el;
+			eo[i] = 0u;
+			ev[i] = 0ull;
+			if(j < lim) {
+				const uint32_t q = partition_of(pref, P, j);
+				eo[i] = q * T + (j - pref[q]);
+				ev[i] = ld_agent(fl + eo[i]);
+			}
+		}
+		uint32_t src[EX_K];
+#pragma unroll
+		for(uint32_t i = 0; i < EX_K; ++i) {
+			const uint32_t j = base + i * nEl + el;
+			src[i] = (j < lim) ? fired_entry(fl + eo[i], ev[i], tag, nd.spinLimit, nd.status, p, eo[i] / T) : 0u;
+		}
+		/* the bucket bounds off[s][p], off[s][p+1], straight into the far
+		 * list slots of the sources (one wave instruction per 64 sources;
+		 * lanes past the end load the row of neuron 0 into slots that are
+		 * not read) */
+#pragma unroll
+		for(uint32_t i = 0; i < EX_K; ++i) {
+			const uint32_t jw = base + i * nEl + wave * 64u;   /* wave-uniform */
+			if(jw < lim) {
+				const NEMO_GLOBAL uint32_t* row = off + size_t(src[i]) * (P + 1u) + p;
+				__builtin_amdgcn_global_load_lds((NEMO_GLOBAL void*) row, (NEMO_LDS void*)(FA + jw), 4, 0, 0);
+				__builtin_amdgcn_global_load_lds((NEMO_GLOBAL void*)(row + 1), (NEMO_LDS void*)(FB + jw), 4, 0, 0);
+			}
+		}
+		__builtin_amdgcn_s_waitcnt(0);
+		/* delay-1 records (first in the bucket, their count in the top bits
+		 * of the start); the rest of the bucket stays in the far list */
+		uint32_t na[EX_K], nn[EX_K];
+		uint32_t R = 0;
+#pragma unroll
+		for(uint32_t i = 0; i < EX_K; ++i) {
+			const uint32_t j = base + i * nEl + el;
+			na[i] = 0u;
+			nn[i] = 0u;
+			if(j < lim) {
+				const uint32_t ar = FA[j];
+				const uint32_t a = ar & offMask;
+				const uint32_t b = FB[j] & offMask;
+				na[i] = a;
+				nn[i] = ar >> OFF_NEAR_SHIFT;
+				events += b - a;
+				FA[j] = a + nn[i];
+				FB[j] = b;
+			}
+			R += nn[i];
+		}
+		for(uint32_t r0 = 0; __any(r0 < R); r0 += EX_NEAR) {   /* wave-uniform */
+			uint2 rec[EX_NEAR];
+#pragma unroll
+			for(uint32_t u = 0; u < EX_NEAR; ++u) {
+				uint32_t r = r0 + u;
+				uint32_t q = 0u;
+				bool found = false;
+#pragma unroll
+				for(uint32_t i = 0; i < EX_K; ++i) {
+					if(!found) {
+						if(r < nn[i]) {
+							q = na[i] + r;
+							found = true;
+						} else {
+							r -= nn[i];
+						}
+					}
+				}
+				rec[u] = ld_rec(syn, found ? q : 0u);
+			}
+#pragma unroll
+			for(uint32_t u = 0; u < EX_NEAR; ++u) {
+				if(r0 + u < R) {
+					ring_add<SAT>(ringE, ringI, ovf, D, T, fwords, s1, rec[u].x & 0xffffu, int32_t(rec[u].y));
+				}
+			}
+		}
+	}
+	/* sources beyond the far list's capacity: every record now */
+	for(uint32_t jb = cap + wave * 128u; jb < F; jb += exw * 128u) {   /* wave-uniform */
+		const uint32_t nsrc = min(128u, F - jb);
+		uint32_t a0 = 0, b0 = 0, a1 = 0, b1 = 0;
+		const uint32_t j0 = jb + lane;
+		const uint32_t j1 = jb + 64u + lane;
+		const bool v0 = lane < nsrc;
+		const bool v1 = 64u + lane < nsrc;
+		const uint32_t q0 = v0 ? partition_of(pref, P, j0) : 0u;
+		const uint32_t q1 = v1 ? partition_of(pref, P, j1) : 0u;
+		const uint32_t o0 = q0 * T + (j0 - pref[q0]);
+		const uint32_t o1 = q1 * T + (j1 - pref[q1]);
+		uint64_t e0 = 0, e1 = 0;
+		if(v0) e0 = ld_agent(fl + o0);
+		if(v1) e1 = ld_agent(fl + o1);
+		uint32_t x0 = 0, x1 = 0;
+		if(v0) x0 = fired_entry(fl + o0, e0, tag, nd.spinLimit, nd.status, p, q0);
+		if(v1) x1 = fired_entry(fl + o1, e1, tag, nd.spinLimit, nd.status, p, q1);
+		if(v0) {
+			const NEMO_GLOBAL uint32_t* o = off + size_t(x0) * (P + 1u) + p;
+			a0 = ld_off(o) & offMask;
+			b0 = ld_off(o + 1) & offMask;
+		}
+		if(v1) {
+			const NEMO_GLOBAL uint32_t* o = off + size_t(x1) * (P + 1u) + p;
+			a1 = ld_off(o) & offMask;
+			b1 = ld_off(o + 1) & offMask;
+		}
+		events += (b0 - a0) + (b1 - a1);
+		deliver_ranges<SAT, IZO>(a0, b0, a1, b1, nsrc, nd.glog2, lane, slot, syn, ringE, ringI, ovf, D, T, fwords);
+	}
+	return events;
+}
+
+/* The far list of an earlier cycle (ranges FA[j]..FB[j] of its Fp fired
+ * sources, relative to that cycle's slot `base`), delivered by nw waves,
+ * wave w taking a contiguous share. Called by whole waves. */
+template<bool SAT, bool IZO>
+__device__ inline void far_phase(const NetDev& nd, uint32_t Fp, const uint32_t* FA, const uint32_t* FB,
+		uint32_t w, uint32_t nw, uint32_t lane, uint32_t base, const NEMO_GLOBAL uint2* syn,
+		int32_t* ringE, int32_t* ringI, uint32_t* ovf, uint32_t fwords)
+{
+	const uint32_t share = (Fp + nw - 1u) / nw;
+	const uint32_t j0 = min(Fp, w * share);
+	const uint32_t j1 = min(Fp, j0 + share);
+	for(uint32_t jb = j0; jb < j1; jb += 128u) {   /* wave-uniform */
+		const uint32_t nsrc = min(128u, j1 - jb);
+		uint32_t a0 = 0, b0 = 0, a1 = 0, b1 = 0;
+		if(lane < nsrc) {
+			a0 = FA[jb + lane];
+			b0 = FB[jb + lane];
+		}
+		if(64u + lane < nsrc) {
+			a1 = FA[jb + 64u + lane];
+			b1 = FB[jb + 64u + lane];
+		}
+		deliver_ranges<SAT, IZO>(a0, b0, a1, b1, nsrc, nd.glog2, lane, base, syn, ringE, ringI, ovf,
+				nd.D, nd.T, fwords);
+	}
+}
+
 /* IZO: every neuron of the network is an Izhikevich neuron, so only its 5
  * parameters and 2 state variables are held in registers. */
 template<bool SAT, bool IZO>
@@ -603,6 +795,17 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 	 * waits for cycle t's exchange. */
 	const bool early = stdp && postLen >= 2u;
 	const bool pring = early && (nd.flags & NEMO_NET_PREF_RING);
+	/* exchange waves: the far list after the optional STDP areas */
+	const uint32_t exw = (defer && nd.exWaves < nwaves) ? nd.exWaves : 0u;
+	const uint32_t fcap = nd.farCap;
+	uint32_t* farA = stdp ? pringArea + (pring ? postLen * PW : 0u) : reinterpret_cast<uint32_t*>(stdpArea);
+	uint32_t* farB = farA + 2u * fcap;
+	uint32_t* fcnt = farB + 2u * fcap;      /* [2], by cycle parity */
+	uint32_t* exFlag = fcnt + 2;
+	if(exw && tid == 0) {
+		fcnt[0] = fcnt[1] = 0u;
+		*exFlag = 0u;
+	}
 	uint32_t rslot = pring ? uint32_t(nd.cycle0 % postLen) : 0u;   /* t mod postLen */
 	if(stdp && tid == 0) {
 		nwork[0] = nwork[1] = 0u;   /* list lengths, by cycle parity */
@@ -755,7 +958,7 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 
 		/* ---- deferred records (delays >= 2) of the previous cycle's spikes,
 		 * delivered while the other partitions publish this cycle */
-		if(defer && kn) {
+		if(!exw && defer && kn) {
 			deliver_ranges<SAT, IZO>(ka0, kb0, ka1, kb1, kn, nd.glog2, lane, (slot + D - 1u) % D, syn,
 					ringE, ringI, ovf, D, T, fwords);
 			kn = 0;
@@ -778,6 +981,31 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 			if(wave != 0) __builtin_amdgcn_s_waitcnt(0);
 		}
 
+		if(exw) {
+			/* ---- 3+4 with exchange waves: the exchange and this cycle's
+			 * delay-1 records on waves [0, exw), the far list of cycle t-1
+			 * on the others (see exchange_phase) */
+			if(early && t >= postLen) {
+				/* the work list is complete and its staged records have
+				 * landed before the accumulation reads them */
+				__builtin_amdgcn_s_waitcnt(0);
+				__syncthreads();
+			}
+			const uint32_t cb = uint32_t(t & 1u);
+			if(wave < exw) {
+				/* the prefix is kept for the list of cycle t+postLen (slot t
+				 * mod postLen, whose cycle t-postLen list was built above) */
+				events += exchange_phase<SAT, IZO>(nd, p, t, k, slot, flb, wave, lane, pref, exFlag,
+						farA + cb * fcap, farB + cb * fcap, fcnt + cb, pring ? pringArea + rslot * PW : nullptr,
+						syn, off, offMask, ringE, ringI, ovf, fwords);
+			} else {
+				far_phase<SAT, IZO>(nd, fcnt[cb ^ 1u], farA + (cb ^ 1u) * fcap, farB + (cb ^ 1u) * fcap,
+						wave - exw, nwaves - exw, lane, (slot + D - 1u) % D, syn, ringE, ringI, ovf, fwords);
+			}
+			if(pring) {
+				rslot = rslot + 1u == postLen ? 0u : rslot + 1u;
+			}
+		} else {
 		/* ---- 3. wait for the network's other partitions */
 		if(wave == 0) {
 			gather_counts(glb(nd.fc) + flb * P, P, uint32_t(t), pref, lane, nd.spinLimit, nd.status, p);
@@ -868,6 +1096,7 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 				}
 			}
 		}
+		}
 
 		/* the row touches above write into `ext`, which the STDP phase and the
 		 * next cycle's stimulus reuse: let them land first */
@@ -938,6 +1167,14 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 						nd.spinLimit, nd.status, p);
 			}
 			if(t >= postLen) {
+				if(exw && wave >= exw) {
+					/* the sources' histories of cycle t are complete only once
+					 * every partition has published the cycle: far waves wait
+					 * for the exchange waves' gather */
+					while(__hip_atomic_load(exFlag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) != k + 1u) {
+						__builtin_amdgcn_s_sleep(1);
+					}
+				}
 				const uint32_t nw = min(nwork[t & 1u], NEMO_STDP_WORK_CAP);
 				const uint32_t ncached = late ? 0u : min(listed, nd.stdpCache);
 				stdp_accumulate<IZO>(work, nw, nwork + 2u + (t & 1u), lane, glb(nd.rfg) + size_t(par) * n_total,
@@ -953,6 +1190,11 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 	}
 
 	/* ---- the last cycle's deferred records */
+	if(exw && nd.nsteps > 0u) {
+		const uint32_t cb = uint32_t((nd.cycle0 + nd.nsteps - 1u) & 1u);
+		far_phase<SAT, IZO>(nd, fcnt[cb], farA + cb * fcap, farB + cb * fcap, wave, nwaves, lane,
+				uint32_t((nd.cycle0 + nd.nsteps - 1u) % D), syn, ringE, ringI, ovf, fwords);
+	}
 	if(defer) {
 		if(kn) {
 			deliver_ranges<SAT, IZO>(ka0, kb0, ka1, kb1, kn, nd.glog2, lane,
@@ -1011,6 +1253,15 @@ nemo_step_kernel(const NetDev* __restrict__ nets, const WorkItem* __restrict__ w
 	run_partition<SAT, IZO>(nd, wi.part);
 }
 
+/* One network per launch: the descriptor travels as a kernel argument, so a
+ * launch needs no copy of it to device memory first. */
+template<bool SAT, bool IZO>
+__global__ void __launch_bounds__(1024)
+nemo_step_kernel_one(const NetDev nd, const WorkItem* __restrict__ work)
+{
+	run_partition<SAT, IZO>(nd, work[blockIdx.x].part);
+}
+
 static const void* step_kernel_variant(uint32_t variant)
 {
 	switch(variant & 3u) {
@@ -1019,6 +1270,42 @@ static const void* step_kernel_variant(uint32_t variant)
 	case 2: return reinterpret_cast<const void*>(&nemo_step_kernel<false, true>);
 	default: return reinterpret_cast<const void*>(&nemo_step_kernel<true, true>);
 	}
+}
+
+static const void* step_kernel_one_variant(uint32_t variant)
+{
+	switch(variant & 3u) {
+	case 0: return reinterpret_cast<const void*>(&nemo_step_kernel_one<false, false>);
+	case 1: return reinterpret_cast<const void*>(&nemo_step_kernel_one<true, false>);
+	case 2: return reinterpret_cast<const void*>(&nemo_step_kernel_one<false, true>);
+	default: return reinterpret_cast<const void*>(&nemo_step_kernel_one<true, true>);
+	}
+}
+
+/* The dynamic LDS limit of a kernel is raised once to the largest size
+ * asked for (the attribute call is a host-side round trip per launch
+ * otherwise). Launches are serialised by the host library's device lock. */
+static hipError_t ensure_lds_limit(const void* kern, uint32_t ldsBytes)
+{
+	static const void* kerns[16] = {};
+	static uint32_t limits[16] = {};
+	for(int i = 0; i < 16; ++i) {
+		if(kerns[i] == kern) {
+			if(limits[i] >= ldsBytes) return hipSuccess;
+			const hipError_t e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, int(ldsBytes));
+			if(e == hipSuccess) limits[i] = ldsBytes;
+			return e;
+		}
+		if(kerns[i] == nullptr) {
+			const hipError_t e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, int(ldsBytes));
+			if(e == hipSuccess) {
+				kerns[i] = kern;
+				limits[i] = ldsBytes;
+			}
+			return e;
+		}
+	}
+	return hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, int(ldsBytes));
 }
 
 /* Batched launch, first node: clear the first 8 words of each network's
@@ -1103,9 +1390,21 @@ int nemo_dev_step_launch(const NetDev* nets_dev, const WorkItem* work_dev, uint3
 		uint32_t blockDim, uint32_t ldsBytes, uint32_t variant, void* stream)
 {
 	const void* kern = step_kernel_variant(variant);
-	hipError_t e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, int(ldsBytes));
+	hipError_t e = ensure_lds_limit(kern, ldsBytes);
 	if(e != hipSuccess) return int(e);
 	void* args[] = { const_cast<NetDev**>(&nets_dev), const_cast<WorkItem**>(&work_dev) };
+	e = hipLaunchCooperativeKernel(kern,
+			dim3(nblocks), dim3(blockDim), args, ldsBytes, static_cast<hipStream_t>(stream));
+	return int(e);
+}
+
+int nemo_dev_step_launch_one(const NetDev* nd_host, const WorkItem* work_dev, uint32_t nblocks,
+		uint32_t blockDim, uint32_t ldsBytes, uint32_t variant, void* stream)
+{
+	const void* kern = step_kernel_one_variant(variant);
+	hipError_t e = ensure_lds_limit(kern, ldsBytes);
+	if(e != hipSuccess) return int(e);
+	void* args[] = { const_cast<NetDev*>(nd_host), const_cast<WorkItem**>(&work_dev) };
 	e = hipLaunchCooperativeKernel(kern,
 			dim3(nblocks), dim3(blockDim), args, ldsBytes, static_cast<hipStream_t>(stream));
 	return int(e);
@@ -1121,7 +1420,7 @@ int nemo_dev_batch_graph_create(const NetDev* nets_dev, uint32_t nnets, const Wo
 {
 	*exec_out = nullptr;
 	const void* kern = step_kernel_variant(variant);
-	hipError_t e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, int(ldsBytes));
+	hipError_t e = ensure_lds_limit(kern, ldsBytes);
 	if(e != hipSuccess) return int(e);
 	hipGraph_t g = nullptr;
 	e = hipGraphCreate(&g, 0);
@@ -1185,7 +1484,7 @@ int nemo_dev_graph_destroy(void* exec)
 int nemo_dev_step_max_blocks(uint32_t blockDim, uint32_t ldsBytes, uint32_t variant, uint32_t* maxBlocks)
 {
 	const void* kern = step_kernel_variant(variant);
-	hipError_t e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, int(ldsBytes));
+	hipError_t e = ensure_lds_limit(kern, ldsBytes);
 	if(e != hipSuccess) return int(e);
 	int perCU = 0;
 	e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kern, int(blockDim), size_t(ldsBytes));

@@ -177,7 +177,7 @@ runBatch(Simulation* const sims[], size_t n, const unsigned nsteps[], unsigned l
 		for(size_t k : members) {
 			Simulation* sim = sims[k];
 			const uint32_t idx = uint32_t(nets.size());
-			nets.push_back(sim->prepareLaunch(nsteps[k], st, false));
+			nets.push_back(sim->prepareLaunch(nsteps[k]));
 			for(uint32_t p = 0; p < sim->partitions(); ++p) items.push_back(std::make_pair(idx, p));
 		}
 		const std::vector<WorkItem> work = Simulation::xcdWorkTable(items);
@@ -199,6 +199,9 @@ runBatch(Simulation* const sims[], size_t n, const unsigned nsteps[], unsigned l
 				events[k] = (unsigned long long) c[2] | ((unsigned long long) c[3] << 32);
 			}
 			try {
+				/* the graph cleared the block before the launch: c counts
+				 * from zero, and the block keeps these values afterwards */
+				sims[k]->setControlBase(c);
 				sims[k]->completeLaunch(nsteps[k], c);
 			} catch(exception& e) {
 				if(!failed) {

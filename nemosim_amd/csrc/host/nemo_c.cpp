@@ -128,6 +128,24 @@ nemo_set_cuda_backend(nemo_configuration_t c, int dev)
 
 nemo_status_t nemo_set_hip_backend(nemo_configuration_t c, int dev) { return nemo_set_cuda_backend(c, dev); }
 
+/* ---- accessors behind the C++ surface (nemo.hpp) that nemo.h lacks */
+
+nemo_configuration_t
+nemo_copy_configuration(nemo_configuration_t c)
+{
+	nemo_configuration_t out = nullptr;
+	CALL(
+		if(!c) throw nemo::exception(NEMO_INVALID_INPUT, "null configuration");
+		out = new nemo_configuration_class(*c);
+	);
+	return out;
+}
+
+nemo_status_t nemo_disable_logging(nemo_configuration_t c) { RETURN_CALL(c->impl.logging = false); }
+nemo_status_t nemo_logging_enabled(nemo_configuration_t c, int* on) { RETURN_CALL(*on = c->impl.logging ? 1 : 0); }
+nemo_status_t nemo_partition_size(nemo_configuration_t c, unsigned* ps) { RETURN_CALL(*ps = c->impl.partitionSize); }
+nemo_status_t nemo_write_only_synapses(nemo_configuration_t c, int* on) { RETURN_CALL(*on = c->impl.writeOnlySynapses ? 1 : 0); }
+
 nemo_status_t nemo_cuda_device(nemo_configuration_t c, int* dev) { RETURN_CALL(*dev = c->impl.device); }
 
 nemo_status_t nemo_backend(nemo_configuration_t c, backend_t* b) { RETURN_CALL(*b = backend_t(c->impl.backend)); }
@@ -157,6 +175,7 @@ nemo_set_delivery_lanes(nemo_configuration_t c, unsigned lanes)
 }
 
 nemo_status_t nemo_set_stdp_lds_staging(nemo_configuration_t c, int enable) { RETURN_CALL(c->impl.stdpLdsStaging = enable != 0); }
+nemo_status_t nemo_set_exchange_waves(nemo_configuration_t c, int waves) { RETURN_CALL(c->impl.exchangeWaves = waves); }
 
 nemo_status_t nemo_set_device_construction(nemo_configuration_t c, int enable) { RETURN_CALL(c->impl.deviceConstruction = enable != 0); }
 
@@ -224,6 +243,30 @@ nemo_add_synapses_csr(nemo_network_t net, size_t nsources, const unsigned source
 }
 
 nemo_status_t nemo_neuron_count(nemo_network_t net, unsigned* n) { RETURN_CALL(*n = net->impl.neuronCount()); }
+
+nemo_status_t nemo_max_delay(nemo_network_t net, unsigned* d) { RETURN_CALL(*d = net->impl.maxDelay()); }
+
+/* src/nemo/NetworkImpl.cpp:28-30,125-127: both start at 0 and follow every
+ * added synapse */
+nemo_status_t
+nemo_min_weight(nemo_network_t net, float* w)
+{
+	RETURN_CALL(
+		float m = 0.0f;
+		for(float x : net->impl.weight()) m = std::min(m, x);
+		*w = m;
+	);
+}
+
+nemo_status_t
+nemo_max_weight(nemo_network_t net, float* w)
+{
+	RETURN_CALL(
+		float m = 0.0f;
+		for(float x : net->impl.weight()) m = std::max(m, x);
+		*w = m;
+	);
+}
 
 /* ------------------------------------------------------------------- simulation */
 

@@ -1,6 +1,7 @@
 #include "simulation.hpp"
 
 #include <algorithm>
+#include <cstdio>
 #include <cstring>
 #include <random>
 #include <dlfcn.h>
@@ -121,6 +122,13 @@ Simulation::Simulation(const NetworkImpl& net, const ConfigurationImpl& conf) :
 	hipStream_t st;
 	hipCheck(hipStreamCreate(&st), "hipStreamCreate");
 	m_stream = st;
+	hipEvent_t ev0, ev1;
+	hipCheck(hipEventCreate(&ev0), "hipEventCreate");
+	m_ev0 = ev0;
+	hipCheck(hipEventCreate(&ev1), "hipEventCreate");
+	m_ev1 = ev1;
+	hipCheck(hipHostMalloc(reinterpret_cast<void**>(&h_ctl), NEMO_CTL_WORDS * 4), "hipHostMalloc");
+	std::memset(h_ctl, 0, NEMO_CTL_WORDS * 4);
 
 	/* ---- simulation-local indices: types in order, insertion order within a
 	 * type (src/nemo/cpu/Simulation.cpp:58-73, src/nemo/cpu/Neurons.cpp:26-39) */
@@ -362,6 +370,10 @@ Simulation::Simulation(const NetworkImpl& net, const ConfigurationImpl& conf) :
 		if(conf.stepMode == NEMO_STEP_FUSED) {
 			throw exception(NEMO_API_UNSUPPORTED, "fused step kernel unavailable: " + whyNotFused);
 		}
+		if(conf.stepMode == NEMO_STEP_AUTO) {
+			/* a different kernel sequence with its own speed: say why */
+			fprintf(stderr, "nemosim HIP: %u neurons run on the plugin step (%s)\n", m_n, whyNotFused.c_str());
+		}
 		/* plugin path: one bucket per source holding all its synapses, with
 		 * absolute targets */
 		m_fused = false;
@@ -550,6 +562,8 @@ Simulation::Simulation(const NetworkImpl& net, const ConfigurationImpl& conf) :
 		}
 		d_syn.upload(syn.data(), syn.size() * sizeof(uint32_t));
 	}
+	planExchange(conf, ldsLimit);
+
 	/* fired lists and counts are kept for flRing cycles: 2 (parity), 3 when
 	 * deferred delivery reads the previous cycle's, postLen + 2 with STDP,
 	 * whose accumulation reads those of cycle t - postLen */
@@ -588,7 +602,6 @@ Simulation::Simulation(const NetworkImpl& net, const ConfigurationImpl& conf) :
 		for(uint32_t p = 0; p < m_P; ++p) items.push_back(std::make_pair(0u, p));
 		std::vector<WorkItem> work = xcdWorkTable(items);
 		d_work.upload(work.data(), work.size() * sizeof(WorkItem));
-		d_net.alloc(sizeof(NetDev));
 	} else {
 		/* pending current as int64 ring [2][D][n], converted inputs, bits */
 		d_acc.alloc(std::max<size_t>(size_t(2) * m_D * m_n, 1) * 8);
@@ -600,6 +613,7 @@ Simulation::Simulation(const NetworkImpl& net, const ConfigurationImpl& conf) :
 		hipCheck(nemo_dev_fill_u64(d_acc.as<uint64_t>(), 0, d_acc.bytes / 8, s), "fill");
 		loadPlugins(net);
 	}
+	hipCheck(hipHostMalloc(reinterpret_cast<void**>(&h_step), std::max<size_t>(m_words, 1) * 8), "hipHostMalloc");
 	hipCheck(hipStreamSynchronize(s), "hipStreamSynchronize");
 	devLock.unlock();
 
@@ -789,6 +803,10 @@ Simulation::~Simulation()
 		(void) hipStreamSynchronize(static_cast<hipStream_t>(m_stream));
 		(void) hipStreamDestroy(static_cast<hipStream_t>(m_stream));
 	}
+	if(m_ev0) (void) hipEventDestroy(static_cast<hipEvent_t>(m_ev0));
+	if(m_ev1) (void) hipEventDestroy(static_cast<hipEvent_t>(m_ev1));
+	if(h_ctl) (void) hipHostFree(h_ctl);
+	if(h_step) (void) hipHostFree(h_step);
 	for(PluginGroup& pg : m_plugins) {
 		if(pg.handle) dlclose(pg.handle);
 	}
@@ -807,17 +825,20 @@ Simulation::localIdx(unsigned g) const
 /* Stimulus schedule to device CSR in simulation-local indices; forced
  * firing is idempotent and a current assignment keeps the last value per
  * neuron and cycle (src/nemo/cpu/Simulation.cpp:140-163). */
-void
-Simulation::uploadSchedule(unsigned nsteps, const nemo_schedule_t* s)
+bool
+Simulation::translateSchedule(unsigned nsteps, const nemo_schedule_t* s, std::vector<uint32_t>& fsPtr,
+		std::vector<uint32_t>& fsIdx, std::vector<uint32_t>& isPtr, std::vector<uint32_t>& isIdx,
+		std::vector<float>& isVal) const
 {
-	std::lock_guard<std::recursive_mutex> devLock(deviceMutex());
-	m_hasSchedule = false;
-	if(!s) return;
-	std::vector<uint32_t> fsPtr, fsIdx, isPtr, isIdx;
-	std::vector<float> isVal;
+	fsPtr.clear();
+	fsIdx.clear();
+	isPtr.clear();
+	isIdx.clear();
+	isVal.clear();
+	if(!s) return false;
 	bool anyF = s->fstim_ptr && s->fstim_idx;
 	bool anyI = s->istim_ptr && s->istim_idx && s->istim_val;
-	if(!anyF && !anyI) return;
+	if(!anyF && !anyI) return false;
 	fsPtr.assign(size_t(nsteps) + 1, 0);
 	isPtr.assign(size_t(nsteps) + 1, 0);
 	std::unordered_map<uint32_t, size_t> last;
@@ -830,7 +851,6 @@ Simulation::uploadSchedule(unsigned nsteps, const nemo_schedule_t* s)
 		fsPtr[k + 1] = uint32_t(fsIdx.size());
 		if(anyI) {
 			last.clear();
-			size_t base = isIdx.size();
 			for(unsigned j = s->istim_ptr[k]; j < s->istim_ptr[k + 1]; ++j) {
 				uint32_t l = localIdx(s->istim_idx[j]);
 				auto it = last.find(l);
@@ -842,10 +862,20 @@ Simulation::uploadSchedule(unsigned nsteps, const nemo_schedule_t* s)
 					isVal[it->second] = s->istim_val[j];
 				}
 			}
-			(void) base;
 		}
 		isPtr[k + 1] = uint32_t(isIdx.size());
 	}
+	return true;
+}
+
+void
+Simulation::uploadSchedule(unsigned nsteps, const nemo_schedule_t* s)
+{
+	std::lock_guard<std::recursive_mutex> devLock(deviceMutex());
+	m_hasSchedule = false;
+	std::vector<uint32_t> fsPtr, fsIdx, isPtr, isIdx;
+	std::vector<float> isVal;
+	if(!translateSchedule(nsteps, s, fsPtr, fsIdx, isPtr, isIdx, isVal)) return;
 	d_fsPtr.ensure(fsPtr.size() * 4);
 	d_isPtr.ensure(isPtr.size() * 4);
 	d_fsIdx.ensure(std::max<size_t>(fsIdx.size(), 1) * 4);
@@ -866,15 +896,49 @@ Simulation::uploadSchedule(unsigned nsteps, const nemo_schedule_t* s)
 	m_hasSchedule = true;
 }
 
+/* Exchange waves (NetDev.exWaves), when the configuration asks for them:
+ * with deferred delivery and LDS left for the far list while the
+ * workgroups stay co-resident, that many waves of each workgroup (at least
+ * one other must remain) run the exchange and the delay-1 records, the rest
+ * the far records of the previous cycle. The far list takes up to 1536
+ * sources per cycle, fewer as LDS allows, and at least 256; the sources of
+ * a cycle beyond it are delivered whole by the exchange waves. */
+void
+Simulation::planExchange(const ConfigurationImpl& conf, uint32_t ldsLimit)
+{
+	m_exWaves = 0;
+	m_farCap = 0;
+	const uint32_t nw = m_T / 64u;
+	if(!m_fused || !m_defer || m_P == 0 || conf.exchangeWaves <= 0) return;
+	const uint32_t ex = uint32_t(conf.exchangeWaves);
+	if(ex + 1u > nw) return;
+	for(uint32_t cap = 1536; cap >= 256; cap -= 64) {
+		const uint32_t lds = m_lds + NEMO_FAR_LDS_BYTES(cap);
+		if(lds > ldsLimit) continue;
+		uint32_t maxBlocks = 0;
+		hipCheck(nemo_dev_step_max_blocks(m_T, lds, variant(), &maxBlocks), "occupancy query");
+		if(m_P <= maxBlocks) {
+			m_exWaves = ex;
+			m_farCap = cap;
+			m_lds = lds;
+			return;
+		}
+	}
+}
+
 NetDev
-Simulation::prepareLaunch(unsigned nsteps, void* stream, bool clearControl)
+Simulation::prepareLaunch(unsigned nsteps)
 {
 	hipCheck(hipSetDevice(m_device), "hipSetDevice");
-	hipStream_t st = static_cast<hipStream_t>(stream);
 	d_raster.ensure(std::max<size_t>(size_t(nsteps) * m_words, 1) * 8);
-	if(clearControl) {
-		hipCheck(nemo_dev_fill_u32(d_ctl.as<uint32_t>(), 0, 16, st), "fill");
-	}
+	return descriptor(nsteps);
+}
+
+/* The launch descriptor: network arrays, per-launch fields, the schedule
+ * uploaded for this launch (if any) and the raster buffer d_raster. */
+NetDev
+Simulation::descriptor(unsigned nsteps) const
+{
 	NetDev nd;
 	std::memset(&nd, 0, sizeof(nd));
 	nd.n = m_n;
@@ -890,6 +954,8 @@ Simulation::prepareLaunch(unsigned nsteps, void* stream, bool clearControl)
 	nd.glog2 = m_glog2;
 	nd.flRing = m_flRing;
 	nd.stdpCache = m_stdpCache;
+	nd.exWaves = m_exWaves;
+	nd.farCap = m_farCap;
 	nd.cycle0 = m_cycle;
 	nd.model = d_model.as<uint8_t>();
 	nd.param = d_param.as<float>();
@@ -921,17 +987,40 @@ Simulation::prepareLaunch(unsigned nsteps, void* stream, bool clearControl)
 	return nd;
 }
 
+/* The control block as a single launch left it (h_ctl, already copied
+ * back): its counts relative to the previous launch's, then the launch's
+ * completion. After an error the block is cleared on the device. */
 void
-Simulation::finishLaunch(unsigned nsteps, void* stream, unsigned long long* events)
+Simulation::readControl(unsigned nsteps, unsigned long long* events)
 {
-	hipStream_t st = static_cast<hipStream_t>(stream);
-	uint32_t ctl[8];
-	hipCheck(hipMemcpyAsync(ctl, d_ctl.ptr, sizeof(ctl), hipMemcpyDeviceToHost, st), "D2H");
-	hipCheck(hipStreamSynchronize(st), "hipStreamSynchronize");
+	uint32_t raw[8], rel[8];
+	std::memcpy(raw, h_ctl, sizeof(raw));
+	std::memcpy(rel, raw, sizeof(rel));
 	unsigned long long e;
-	std::memcpy(&e, &ctl[2], sizeof(e));
+	std::memcpy(&e, &raw[2], sizeof(e));
+	e -= m_ctlEvents;
+	std::memcpy(&rel[2], &e, sizeof(e));
+	rel[0] = raw[0] - m_ctlStarted;
+	rel[7] = raw[7] - m_ctlStarted;
 	if(events) *events = e;
-	completeLaunch(nsteps, ctl);
+	setControlBase(raw);
+	completeLaunch(nsteps, rel);
+}
+
+void
+Simulation::setControlBase(const uint32_t ctl[8])
+{
+	if(ctl[1] != 0u) {
+		/* a failed launch: start the next one from a clear block */
+		hipStream_t st = static_cast<hipStream_t>(m_stream);
+		hipCheck(nemo_dev_fill_u32(d_ctl.as<uint32_t>(), 0, NEMO_CTL_WORDS, st), "fill");
+		hipCheck(hipStreamSynchronize(st), "hipStreamSynchronize");
+		m_ctlStarted = 0;
+		m_ctlEvents = 0;
+		return;
+	}
+	m_ctlStarted = ctl[0];
+	std::memcpy(&m_ctlEvents, &ctl[2], sizeof(m_ctlEvents));
 }
 
 void
@@ -991,9 +1080,7 @@ Simulation::launchPlugins(unsigned nsteps, unsigned long long* events)
 		s.is_idx = d_isIdx.as<uint32_t>();
 		s.is_val = d_isVal.as<float>();
 	}
-	hipEvent_t e0, e1;
-	hipCheck(hipEventCreate(&e0), "hipEventCreate");
-	hipCheck(hipEventCreate(&e1), "hipEventCreate");
+	hipEvent_t e0 = static_cast<hipEvent_t>(m_ev0), e1 = static_cast<hipEvent_t>(m_ev1);
 	hipCheck(hipEventRecord(e0, st), "hipEventRecord");
 	if(m_replay) {
 		/* weights changed: rebuild the pending current from the history */
@@ -1022,13 +1109,13 @@ Simulation::launchPlugins(unsigned nsteps, unsigned long long* events)
 	}
 	hipCheck(hipEventRecord(e1, st), "hipEventRecord");
 	uint32_t ctl[8];
-	hipCheck(hipMemcpyAsync(ctl, d_ctl.ptr, sizeof(ctl), hipMemcpyDeviceToHost, st), "D2H");
+	hipCheck(hipMemcpyAsync(h_ctl, d_ctl.ptr, sizeof(ctl), hipMemcpyDeviceToHost, st), "D2H");
 	hipCheck(hipStreamSynchronize(st), "hipStreamSynchronize");
+	std::memcpy(ctl, h_ctl, sizeof(ctl));
+	setControlBase(ctl);
 	float ms = 0.0f;
 	hipCheck(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
 	m_kernelMs = ms;
-	(void) hipEventDestroy(e0);
-	(void) hipEventDestroy(e1);
 	m_cycle += nsteps;
 	m_timerCycles += nsteps;
 	m_lastRunSteps = nsteps;
@@ -1056,23 +1143,37 @@ Simulation::launch(unsigned nsteps, unsigned long long* events)
 		m_hasSchedule = false;
 		return;
 	}
+	launchFused(nsteps, false);
+	readControl(nsteps, events);
+}
+
+/* One launch of the fused kernel: the descriptor goes by value, HIP events
+ * on the kernel's own stream time the kernel alone, and one copy brings back
+ * the control block (and, for a single step, its raster words) into pinned
+ * memory, followed by the launch's only synchronisation. */
+void
+Simulation::launchFused(unsigned nsteps, bool stepOut)
+{
+	launchDescriptor(prepareLaunch(nsteps), stepOut);
+}
+
+void
+Simulation::launchDescriptor(const NetDev& nd, bool stepOut)
+{
 	hipStream_t st = static_cast<hipStream_t>(m_stream);
-	NetDev nd = prepareLaunch(nsteps, m_stream);
-	hipCheck(hipMemcpyAsync(d_net.ptr, &nd, sizeof(nd), hipMemcpyHostToDevice, st), "H2D");
-	/* HIP events on the kernel's own stream time the step kernel alone */
-	hipEvent_t e0, e1;
-	hipCheck(hipEventCreate(&e0), "hipEventCreate");
-	hipCheck(hipEventCreate(&e1), "hipEventCreate");
+	hipEvent_t e0 = static_cast<hipEvent_t>(m_ev0), e1 = static_cast<hipEvent_t>(m_ev1);
 	hipCheck(hipEventRecord(e0, st), "hipEventRecord");
-	hipCheck(nemo_dev_step_launch(d_net.as<NetDev>(), d_work.as<WorkItem>(), m_P, m_T, m_lds, variant(), st),
+	hipCheck(nemo_dev_step_launch_one(&nd, d_work.as<WorkItem>(), m_P, m_T, m_lds, variant(), st),
 			"step kernel launch");
 	hipCheck(hipEventRecord(e1, st), "hipEventRecord");
-	finishLaunch(nsteps, m_stream, events);
+	hipCheck(hipMemcpyAsync(h_ctl, d_ctl.ptr, 32, hipMemcpyDeviceToHost, st), "D2H");
+	if(stepOut && m_words) {
+		hipCheck(hipMemcpyAsync(h_step, d_raster.ptr, size_t(m_words) * 8, hipMemcpyDeviceToHost, st), "D2H");
+	}
+	hipCheck(hipStreamSynchronize(st), "hipStreamSynchronize");
 	float ms = 0.0f;
 	hipCheck(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
 	m_kernelMs = ms;
-	(void) hipEventDestroy(e0);
-	(void) hipEventDestroy(e1);
 }
 
 void
@@ -1104,6 +1205,13 @@ Simulation::step(const std::vector<unsigned>& fstim, const unsigned* istimIdx, c
 	if(fstim.empty()) { s.fstim_ptr = nullptr; }
 	if(nistim == 0) { s.istim_ptr = nullptr; s.istim_idx = nullptr; s.istim_val = nullptr; }
 	uploadSchedule(1, &s);
+	if(m_fused && m_P > 0 && m_n > 0) {
+		/* the raster words come back with the control block */
+		launchFused(1, true);
+		readControl(1, nullptr);
+		decodeStep(h_step, m_fired);
+		return m_fired;
+	}
 	launch(1, nullptr);
 	m_rasterHost.resize(m_words);
 	if(m_words) {
@@ -1152,7 +1260,7 @@ Simulation::readRaster(uint64_t words[], size_t cap, size_t* wordsPerStep)
 	size_t nw = size_t(m_lastRunSteps) * m_words;
 	if(wordsPerStep) *wordsPerStep = m_words;
 	if(words) {
-		if(cap < nw) throw exception(NEMO_BUFFER_OVERFLOW, "raster buffer too small");
+			if(cap < nw) throw exception(NEMO_BUFFER_OVERFLOW, "raster buffer too small");
 		if(nw) hipCheck(hipMemcpy(words, d_raster.ptr, nw * 8, hipMemcpyDeviceToHost), "D2H");
 	}
 }
@@ -1177,6 +1285,8 @@ Simulation::info(nemo_hip_info_t* out) const
 	out->lanes_per_source = m_fused ? (1u << m_glog2) : 0u;
 	out->stdp_staged_chunks = m_stdpCache;
 	out->stdp_prefix_ring = m_prefRing ? 1u : 0u;
+	out->exchange_waves = m_exWaves;
+	out->far_list_sources = m_farCap;
 	out->device_built = m_deviceBuilt ? 1u : 0u;
 }
 

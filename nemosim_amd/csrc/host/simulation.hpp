@@ -32,6 +32,7 @@ struct ConfigurationImpl {
 	bool stdpLdsStaging = true;   /* STDP count prefixes and reverse records staged in LDS */
 	bool deviceConstruction = true;   /* synapse structures built on the device */
 	int stepMode = NEMO_STEP_AUTO;
+	int exchangeWaves = 0;        /* 0 = automatic, < 0 = none */
 	std::string description;
 
 	/* src/nemo/StdpFunction.cpp:12-50 */
@@ -73,6 +74,7 @@ class Simulation
 
 		/* multi-cycle runs */
 		void run(unsigned nsteps, const nemo_schedule_t* schedule, unsigned long long* events);
+
 		void readFired(unsigned counts[], unsigned ids[], size_t cap, size_t* total);
 		void readRaster(uint64_t words[], size_t cap, size_t* wordsPerStep);
 		void info(nemo_hip_info_t* out) const;
@@ -100,13 +102,16 @@ class Simulation
 		uint32_t blockThreads() const { return m_T; }
 		uint32_t ldsBytes() const { return m_lds; }
 		/* prepare a launch of nsteps cycles: fills and returns the device
-		 * descriptor (host copy) with per-launch fields set; clears the
-		 * status word and event counter unless the caller's launch does */
-		NetDev prepareLaunch(unsigned nsteps, void* stream, bool clearControl = true);
-		/* read back status and events after the launch, then completeLaunch */
-		void finishLaunch(unsigned nsteps, void* stream, unsigned long long* events);
-		/* advance the cycle count by a finished launch; throws on a status error */
+		 * descriptor (host copy) with per-launch fields set */
+		NetDev prepareLaunch(unsigned nsteps);
+		/* advance the cycle count by a finished launch; throws on a status
+		 * error. ctl holds the first 8 words of the control block as the
+		 * launch left them, counted from zero at the launch's start (a
+		 * batched launch clears them first). */
 		void completeLaunch(unsigned nsteps, const uint32_t ctl[8]);
+		/* the control block's words as a batched launch left them on the
+		 * device: the base from which the next single launch counts */
+		void setControlBase(const uint32_t ctl[8]);
 		unsigned neuronCount() const { return m_n; }
 		int device() const { return m_device; }
 		/* step-kernel variant (nemo_step_variant): saturating, Izhikevich-only */
@@ -153,6 +158,9 @@ class Simulation
 		uint32_t m_flRing = 2;
 		uint32_t m_stdpCache = 0;    /* STDP chunks staged in LDS per cycle */
 		bool m_prefRing = false;     /* STDP list prefixes of postLen cycles kept in LDS */
+		uint32_t m_exWaves = 0;      /* exchange waves (NetDev.exWaves) */
+		uint32_t m_farCap = 0;       /* far-list capacity in sources (NetDev.farCap) */
+		void planExchange(const ConfigurationImpl& conf, uint32_t ldsLimit);
 		/* every neuron is an Izhikevich neuron (NEMO_NET_IZHIKEVICH_ONLY) */
 		bool m_izOnly = false;
 
@@ -179,9 +187,31 @@ class Simulation
 		DevBuf d_rcmStart, d_rcmSd, d_rcmFwd, d_rcmAcc, d_stdp;
 		DevBuf d_fsPtr, d_fsIdx, d_isPtr, d_isIdx, d_isVal;
 		DevBuf d_raster;
-		DevBuf d_net, d_work;
+		DevBuf d_work;
 		size_t m_deviceBytes = 0;
 		float m_kernelMs = 0.0f;
+		/* HIP events around each step-kernel launch (created once) */
+		void* m_ev0 = nullptr;
+		void* m_ev1 = nullptr;
+		/* pinned host staging: [0..15] the control block read back after a
+		 * launch, then one cycle's raster words (nemo_step) */
+		uint32_t* h_ctl = nullptr;
+		uint64_t* h_step = nullptr;
+		/* control-block words 0 (workgroups started) and 2..3 (delivered
+		 * synapses) as the last launch left them: the device block is not
+		 * cleared between single launches, each launch's counts are the
+		 * difference */
+		uint32_t m_ctlStarted = 0;
+		unsigned long long m_ctlEvents = 0;
+		/* the step kernel's launch: descriptor by value, no copies before it */
+		void launchFused(unsigned nsteps, bool stepOut);
+		void launchDescriptor(const NetDev& nd, bool stepOut);
+		NetDev descriptor(unsigned nsteps) const;
+		bool translateSchedule(unsigned nsteps, const nemo_schedule_t* s, std::vector<uint32_t>& fsPtr,
+				std::vector<uint32_t>& fsIdx, std::vector<uint32_t>& isPtr, std::vector<uint32_t>& isIdx,
+				std::vector<float>& isVal) const;
+
+		void readControl(unsigned nsteps, unsigned long long* events);
 
 		/* schedule of the current launch (host side, sim-local indices) */
 		bool m_hasSchedule = false;

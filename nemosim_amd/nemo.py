@@ -100,6 +100,11 @@ class Configuration(object):
         """STDP count prefixes and reverse records staged in LDS (default on; same results)."""
         _check(self._lib.nemo_set_stdp_lds_staging(self._h, 1 if enable else 0))
 
+    def set_exchange_waves(self, waves):
+        """Waves per step-kernel workgroup running the exchange and delay-1
+        delivery (0 or < 0 = none, the default; same results)."""
+        _check(self._lib.nemo_set_exchange_waves(self._h, int(waves)))
+
     def set_device_construction(self, enable):
         """Synapse structures built on the device (default) or on the host; same results."""
         _check(self._lib.nemo_set_device_construction(self._h, 1 if enable else 0))

@@ -84,18 +84,20 @@ class Request(object):
         return 30 * n * m + 12 * n * (p + 1) + (1 << 20)
 
 
-def build_simulation(req, device_construction=False, tuning=None):
+def build_simulation(req, device_construction=False, tuning=None, synapse_queries=False):
     """The request's simulation. Built on the host by default: a serving
     worker then needs the device only for the final upload, while a device
     build holds the device lock between chunks (measured on 200 Poisson
     requests: p50 latency 1.8 s host-built, 3.1 s device-built). With a
     tuning table (nemosim_amd.autotune.load_table()) the partition size and
     delivery lanes come from it instead of the backend's defaults; results
-    do not depend on either."""
+    do not depend on either. Synapse queries (getSynapseWeight ...) are off
+    unless synapse_queries: their host tables are not kept."""
     spec = req.spec()
     net = spec.to_nemo(nemo_api)
     conf = nemo_api.Configuration()
-    conf.set_write_only_synapses()
+    if not synapse_queries:
+        conf.set_write_only_synapses()
     conf.set_device_construction(device_construction)
     if tuning:
         T = autotune.partition_size(req.neurons, tuning)
@@ -129,7 +131,11 @@ def poisson_requests(count, rate_hz, seed, size_range=(1000, 50000), syn_range=(
 
 class Server(object):
     def __init__(self, hbm_budget_bytes, chunk_cycles=100, build_workers=4, keep_rasters=True,
-                 device_construction=False, tuning=None):
+                 device_construction=False, tuning=None, keep_finished=False):
+        """keep_finished: a finished request keeps its simulation (built
+        with synapse queries on) for inspection after serve(); its budget
+        is released all the same."""
+        self.keep_finished = keep_finished
         self.budget = int(hbm_budget_bytes)
         self.device_construction = device_construction
         self.tuning = tuning
@@ -198,7 +204,8 @@ class Server(object):
                 waiting.pop(0)
                 self.used += need
                 r.admitted_s = now()
-                building.append((self.pool.submit(build_simulation, r, self.device_construction, self.tuning), r))
+                building.append((self.pool.submit(build_simulation, r, self.device_construction, self.tuning,
+                                                  self.keep_finished), r))
             # built networks join the resident set
             still = []
             for fut, r in building:
@@ -214,7 +221,8 @@ class Server(object):
                 finished = [r for r in active if r.cycles_done >= r.duration_ms]
                 for r in finished:
                     r.done_s = now()
-                    r.sim = None
+                    if not self.keep_finished:
+                        r.sim = None
                     self.used -= r.estimated_device_bytes()
                     done.append(r)
                 active = [r for r in active if r.cycles_done < r.duration_ms]
@@ -246,9 +254,11 @@ def raster_of(req):
     return c, i
 
 
-def run_alone(req):
-    """The same request on its own (for the alone-vs-mix identity)."""
-    sim = build_simulation(req)
+def run_alone(req, keep_sim=False):
+    """The same request on its own (for the alone-vs-mix identity): its
+    (counts, ids), and with keep_sim also the simulation (built with
+    synapse queries on)."""
+    sim = build_simulation(req, synapse_queries=keep_sim)
     counts, ids = [], []
     done = 0
     while done < req.duration_ms:
@@ -262,4 +272,6 @@ def run_alone(req):
         done += k
         if req.stdp and done % req.stdp_period == 0:
             sim.apply_stdp(req.reward)
+    if keep_sim:
+        return np.concatenate(counts), np.concatenate(ids), sim
     return np.concatenate(counts), np.concatenate(ids)

@@ -961,6 +961,17 @@ int oracle_num_threads()
 #endif
 }
 
+/* Threads of the calling host thread's parallel loops (one oracle per host
+ * thread can then run many small networks side by side). */
+void oracle_set_threads(int n)
+{
+#ifdef _OPENMP
+	omp_set_num_threads(n > 0 ? n : 1);
+#else
+	(void) n;
+#endif
+}
+
 int oracle_add_neurons(void* o, unsigned model, const unsigned* gidx, size_t n,
 		unsigned nargs, const float* args)
 {

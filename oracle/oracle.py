@@ -39,6 +39,7 @@ def load():
         "oracle_new": (ctypes.c_void_p, []),
         "oracle_delete": (None, [ctypes.c_void_p]),
         "oracle_num_threads": (ctypes.c_int, []),
+        "oracle_set_threads": (None, [ctypes.c_int]),
         "oracle_add_neurons": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint, _c_uint_p, ctypes.c_size_t,
                                               ctypes.c_uint, _c_float_p]),
         "oracle_add_synapses": (ctypes.c_int, [ctypes.c_void_p, _c_uint_p, _c_uint_p, _c_uint_p, _c_float_p,
@@ -270,6 +271,63 @@ def from_spec(spec, stdp=None):
         o.set_stdp_function(*stdp)
     o.finalize()
     return o
+
+
+def set_threads(n):
+    """OpenMP threads of the oracle's parallel loops when called from the
+    calling Python thread (ctypes releases the GIL, so several oracles can
+    run side by side, one per thread, each with n threads)."""
+    load().oracle_set_threads(int(n))
+
+
+def map_parallel(fn, items, workers=None, threads_each=1, on_done=None):
+    """[fn(x) for x in items], run on a pool of host threads, each of whose
+    oracles uses threads_each OpenMP threads. For many small networks: one
+    network per thread is faster than one at a time across all cores.
+    on_done(done_count, total), if given, is called on the calling thread as
+    results come in."""
+    import concurrent.futures
+    items = list(items)
+    workers = workers or max(1, min(16, os.cpu_count() or 1) // max(1, threads_each))
+
+    def call(x):
+        set_threads(threads_each)
+        return fn(x)
+
+    out = [None] * len(items)
+    with concurrent.futures.ThreadPoolExecutor(max_workers=workers) as pool:
+        futs = {pool.submit(call, x): i for i, x in enumerate(items)}
+        done = 0
+        for f in concurrent.futures.as_completed(futs):
+            out[futs[f]] = f.result()
+            done += 1
+            if on_done is not None:
+                on_done(done, len(items))
+    return out
+
+
+def first_divergence(ref_counts, ref_ids, got_counts, got_ids):
+    """(cycle, detail) of the first cycle whose fired set differs, or None.
+    Rasters are per-cycle fired counts plus the fired ids back to back."""
+    ref_counts = np.asarray(ref_counts, np.int64)
+    got_counts = np.asarray(got_counts, np.int64)
+    ro = np.concatenate([[0], np.cumsum(ref_counts)]).astype(np.int64)
+    go = np.concatenate([[0], np.cumsum(got_counts)]).astype(np.int64)
+    ref_ids = np.asarray(ref_ids, np.int64)
+    got_ids = np.asarray(got_ids, np.int64)
+    if len(ref_counts) != len(got_counts):
+        return min(len(ref_counts), len(got_counts)), "raster lengths differ: %d and %d cycles" % (
+            len(ref_counts), len(got_counts))
+    if np.array_equal(ref_counts, got_counts) and np.array_equal(ref_ids, got_ids):
+        return None
+    for t in range(len(ref_counts)):
+        a = ref_ids[ro[t]:ro[t + 1]]
+        b = got_ids[go[t]:go[t + 1]]
+        if len(a) != len(b) or not np.array_equal(a, b):
+            extra = sorted(set(b.tolist()) - set(a.tolist()))[:5]
+            missing = sorted(set(a.tolist()) - set(b.tolist()))[:5]
+            return t, "ref %d fired, got %d; extra %s missing %s" % (len(a), len(b), extra, missing)
+    return None
 
 
 def fx_toFix(f):

@@ -72,15 +72,5 @@ class HipBuilder(object):
 
 def first_divergence(ref_counts, ref_ids, got_counts, got_ids):
     """(cycle, detail) of the first cycle whose fired set differs, or None."""
-    ro = np.concatenate([[0], np.cumsum(np.asarray(ref_counts, np.int64))]).astype(np.int64)
-    go = np.concatenate([[0], np.cumsum(np.asarray(got_counts, np.int64))]).astype(np.int64)
-    ref_ids = np.asarray(ref_ids, np.int64)
-    got_ids = np.asarray(got_ids, np.int64)
-    for t in range(len(ref_counts)):
-        a = ref_ids[ro[t]:ro[t + 1]]
-        b = got_ids[go[t]:go[t + 1]]
-        if len(a) != len(b) or not np.array_equal(a, b):
-            extra = sorted(set(b.tolist()) - set(a.tolist()))[:5]
-            missing = sorted(set(a.tolist()) - set(b.tolist()))[:5]
-            return t, "ref %d fired, got %d; extra %s missing %s" % (len(a), len(b), extra, missing)
-    return None
+    import oracle
+    return oracle.first_divergence(ref_counts, ref_ids, got_counts, got_ids)

@@ -14,6 +14,16 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "slow: long-running parity case")
 
 
+@pytest.fixture
+def progress(capsys):
+    """Progress lines of a long test, written to the terminal as they happen
+    (not held back by output capture)."""
+    def say(msg):
+        with capsys.disabled():
+            print(msg, flush=True)
+    return say
+
+
 @pytest.fixture(scope="session")
 def nemo_mod():
     import nemosim_amd

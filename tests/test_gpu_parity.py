@@ -2,7 +2,6 @@
 cycle (first diverging cycle reported), neuron state bit-exact (the FP
 arithmetic keeps NeMo's op order; non-finite values compare by class), and
 synapse weights bit-exact under STDP."""
-import os
 
 import numpy as np
 import pytest
@@ -148,6 +147,42 @@ def test_stdp_parity(nemo_mod, oracle_mod):
         exp = np.float32(np.float32(ref_w[k]) / np.float32(1 << 20))
         assert got == exp, (k, got, exp)
     assert np.any(ref_w != np.array([int(np.trunc(np.float64(w) * (1 << 20))) for w in spec.weights], np.int64))
+
+
+@pytest.mark.parametrize("stdp_on", [False, True])
+@pytest.mark.parametrize("waves", [-1, 1, 3, 7])
+def test_exchange_waves(nemo_mod, oracle_mod, waves, stdp_on):
+    """Exchange waves (the exchange and the delay-1 records on `waves` waves
+    of each 512-neuron workgroup, the previous cycle's far records on the
+    others; -1: none), with and without STDP, in launches of uneven length
+    and with applyStdp between them, against the oracle: rasters, state and
+    weights."""
+    spec = examples.izhikevich_80_20(6000, 300, seed=29, stdp=stdp_on)
+    stdp = None
+    if stdp_on:
+        pre, post = examples.stdp_window()
+        stdp = (pre, post, 0.0, 1.0, 0.0, -1.0)
+    net = spec.to_nemo(nemo_mod)
+    conf = nemo_mod.Configuration()
+    if stdp:
+        conf.set_stdp_function(stdp[0], stdp[1], min_e=0.0, max_e=1.0, min_i=0.0, max_i=-1.0)
+    conf.set_partition_size(512)
+    conf.set_exchange_waves(waves)
+    sim = nemo_mod.Simulation(net, conf)
+    info = sim.info()
+    assert info["deferred"] == 1
+    assert info["exchange_waves"] == max(waves, 0), info
+    if waves > 0:
+        assert info["far_list_sources"] >= 256, info
+    orc = oracle_mod.from_spec(spec, stdp=stdp)
+    for k in [1, 37, 250, 112]:
+        compare_run(sim, orc, k)
+        if stdp_on:
+            sim.apply_stdp(1.0)
+            orc.apply_stdp(1.0)
+    compare_state(sim, orc, spec.n, 2, sample=range(0, spec.n, 11))
+    if stdp_on:
+        compare_weights(sim, orc, spec, 5000, seed=3)
 
 
 @pytest.mark.parametrize("staging", [True, False])
@@ -574,13 +609,8 @@ def test_c3_stdp_epochs(nemo_mod, oracle_mod):
 
 
 
-LONG_PARITY = os.environ.get("NEMOSIM_LONG_PARITY") == "1"
-LONG_REASON = "minutes of oracle time on the host: set NEMOSIM_LONG_PARITY=1 to run"
-
-
 @pytest.mark.slow
-@pytest.mark.skipif(not LONG_PARITY, reason=LONG_REASON)
-def test_c2_10000_cycles(nemo_mod, oracle_mod):
+def test_c2_10000_cycles(nemo_mod, oracle_mod, progress):
     """The 200k x 1000 network for 10000 cycles (1.5e10 synaptic events):
     the full raster bit-exact against the oracle, 1000 cycles at a time,
     then a sample of the neuron state."""
@@ -589,13 +619,12 @@ def test_c2_10000_cycles(nemo_mod, oracle_mod):
     orc = oracle_mod.from_spec(spec)
     for block in range(10):
         rc, _ = compare_run(sim, orc, 1000)
-        print("C2 cycles %d-%d identical, %d spikes" % (1000 * block, 1000 * block + 999, int(rc.sum())), flush=True)
+        progress("C2 cycles %d-%d identical, %d spikes" % (1000 * block, 1000 * block + 999, int(rc.sum())))
     compare_state(sim, orc, spec.n, 2, sample=range(0, spec.n, 97))
 
 
 @pytest.mark.slow
-@pytest.mark.skipif(not LONG_PARITY, reason=LONG_REASON)
-def test_c3_60_epochs(nemo_mod, oracle_mod):
+def test_c3_60_epochs(nemo_mod, oracle_mod, progress):
     """The 50k x 1000 STDP network for 60 simulated seconds with
     applyStdp(1.0) after each: every second's raster bit-exact against the
     oracle, then a sample of 200000 weights."""
@@ -609,5 +638,5 @@ def test_c3_60_epochs(nemo_mod, oracle_mod):
         rc, _ = compare_run(sim, orc, 1000)
         sim.apply_stdp(1.0)
         orc.apply_stdp(1.0)
-        print("C3 second %d identical, %d spikes" % (epoch, int(rc.sum())), flush=True)
+        progress("C3 second %d identical, %d spikes" % (epoch, int(rc.sum())))
     compare_weights(sim, orc, spec, 200000, seed=5)
