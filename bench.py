@@ -19,7 +19,8 @@ backend) timed on a bounded sample of the same workload on this host.
 Other workloads (--workload): "stdp" (50k x 1000, STDP, applyStdp every
 1000 cycles), "batch" (256 networks of 1k-20k neurons, batched persistent
 launch vs one plain run per network), "serve" (Poisson requests under an
-HBM budget, p50/p99 latency).
+HBM budget, p50/p99 latency), "step" (NeMo's per-cycle nemo_step at
+1k x 100 and 200k x 1000).
 """
 import argparse
 import json
@@ -281,6 +282,68 @@ def bench_plain(args, stdp=False):
     return result
 
 
+def bench_step(args):
+    """NeMo's per-cycle API: nemo_step (one 1 ms cycle per call, the fired
+    ids returned to the host) at the 1k x 100 and 200k x 1000 sizes, called
+    through the C ABI with no Python work in the timed loop beyond copying
+    the fired ids into a preallocated buffer."""
+    import ctypes
+    import nemosim_amd
+    from nemosim_amd import examples, _lib
+    lib = _lib.load()
+    lines = []
+    for n, m in [(1000, 100), (200000, 1000)]:
+        spec = examples.izhikevich_80_20(n, m, seed=args.seed)
+        sim = build_sim(nemosim_amd, spec)
+        warm, steps = args.warmup, args.steps
+        counts = np.zeros(warm + steps, np.uint32)
+        ids = np.zeros((warm + steps) * 64 + n, np.uint32)
+        fired = ctypes.POINTER(ctypes.c_uint)()
+        nf = ctypes.c_size_t()
+        pos = 0
+        wall = 0.0
+        for t in range(warm + steps):
+            t0 = time.perf_counter()
+            st = lib.nemo_step(sim._h, None, 0, None, None, 0, ctypes.byref(fired), ctypes.byref(nf))
+            k = nf.value
+            if pos + k > len(ids):
+                ids = np.concatenate([ids, np.zeros(len(ids), np.uint32)])
+            if k:
+                ctypes.memmove(ids.ctypes.data + 4 * pos, fired, 4 * k)
+            if t >= warm:
+                wall += time.perf_counter() - t0
+            if st != 0:
+                raise RuntimeError(lib.nemo_strerror().decode())
+            counts[t] = k
+            pos += k
+        deg, _ = delay_tables(spec)
+        timed_ids = ids[int(counts[:warm].sum()):pos]
+        events = int(deg[timed_ids.astype(np.int64)].sum())
+        run = {"counts": counts, "ids": ids[:pos], "chunks": [(1, False)] * (warm + steps), "timed": (warm, warm + steps)}
+        parity, _ = oracle_leg(spec, run, 0.0, args.parity_seconds, baseline=False) if not args.no_parity else (None, None)
+        lines.append({"neurons": n, "synapses_per_neuron": m, "steps": steps, "warmup": warm,
+                      "ms_per_step": wall * 1e3 / steps, "ms_per_sim_second": wall * 1e3 / (steps / 1000.0),
+                      "synaptic_events_per_s": events / wall if wall > 0 else None,
+                      "kernel_ms_last_step": sim.info()["last_kernel_ms"], "parity": parity})
+        log("nemo_step %d x %d: %.1f us per step" % (n, m, wall * 1e6 / steps))
+        del sim
+    return {
+        "metric": "wall ms per nemo_step (one 1 ms cycle per call)",
+        "value": lines[-1]["ms_per_step"],
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "q11.20-i32/f32/f64",
+        "data": "synthetic (seeded Izhikevich 80/20 construction)",
+        "config": {"workload": "nemo_step per cycle at 1k x 100 and 200k x 1000, STDP off"},
+        "sizes": lines,
+    }
+
+
 def traffic_from_profile(path):
     """HBM bytes per step-kernel launch from the committed rocprof counter
     summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration),
@@ -396,7 +459,8 @@ def copy_calibration():
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", default="plain", choices=["plain", "stdp", "batch", "serve", "copy-calibration"])
+    ap.add_argument("--workload", default="plain",
+                    choices=["plain", "stdp", "batch", "serve", "step", "copy-calibration"])
     ap.add_argument("--steps", type=int, default=5000, help="timed cycles")
     ap.add_argument("--warmup", type=int, default=1000, help="warm-up cycles")
     ap.add_argument("--launch-steps", type=int, default=1000, help="cycles per persistent launch")
@@ -433,6 +497,8 @@ def main():
         result = bench_plain(args, stdp=True)
     elif args.workload == "batch":
         result = bench_batch(args)
+    elif args.workload == "step":
+        result = bench_step(args)
     elif args.workload == "copy-calibration":
         result = copy_calibration()
     else:

@@ -719,11 +719,27 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 		atomicAdd(nd.status - 1, 1u);
 	}
 
-	/* restore the pending-current ring */
+	/* restore the pending-current ring: 1 KB per wave instruction straight
+	 * into LDS (16 bytes per lane), every load in flight at once; the ring
+	 * is 8 D T bytes, a multiple of 512 since T is a multiple of 64, so at
+	 * most half a KB is left for ordinary loads */
 	const int32_t* gring = nd.ring + size_t(p) * 2u * D * T;
 	const bool replay = (nd.flags & NEMO_NET_REPLAY) != 0u;
-	for(uint32_t i = tid; i < 2u * D * T; i += nthreads) {
-		ringE[i] = replay ? 0 : gring[i];
+	const uint32_t ringWords = 2u * D * T;
+	if(replay) {
+		for(uint32_t i = tid; i < ringWords; i += nthreads) {
+			ringE[i] = 0;
+		}
+	} else {
+		const uint32_t chunks = ringWords / 256u;
+		for(uint32_t c = wave; c < chunks; c += nwaves) {   /* wave-uniform */
+			__builtin_amdgcn_global_load_lds((NEMO_GLOBAL void*)(gring + c * 256u + lane * 4u),
+					(NEMO_LDS void*)(ringE + c * 256u), 16, 0, 0);
+		}
+		for(uint32_t i = chunks * 256u + tid; i < ringWords; i += nthreads) {
+			ringE[i] = gring[i];
+		}
+		__builtin_amdgcn_s_waitcnt(0);
 	}
 	if(SAT) {
 		for(uint32_t i = tid; i < 2u * D * fwords; i += nthreads) {
@@ -1224,8 +1240,11 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 			gout[D * T + i] = ii;
 		}
 	} else {
-		for(uint32_t i = tid; i < 2u * D * T; i += nthreads) {
-			gout[i] = ringE[i];
+		/* 16 bytes per lane and instruction (both sides 16-byte aligned) */
+		const uint4* src4 = reinterpret_cast<const uint4*>(ringE);
+		NEMO_GLOBAL uint4* dst4 = glb(reinterpret_cast<uint4*>(gout));
+		for(uint32_t i = tid; i < ringWords / 4u; i += nthreads) {
+			dst4[i] = src4[i];
 		}
 	}
 	/* delivered-synapse count */

@@ -338,6 +338,14 @@ nemo_hip_run_batch(nemo_simulation_t sims[], size_t n, const unsigned nsteps[], 
 {
 	RETURN_CALL(
 		if(n == 0) return NEMO_OK;
+		if(!sims) throw nemo::exception(NEMO_INVALID_INPUT, "null simulation array");
+		/* a simulation listed twice would share its state and exchange
+		 * buffers between two sets of workgroups of one launch */
+		std::vector<nemo_simulation_t> sorted(sims, sims + n);
+		std::sort(sorted.begin(), sorted.end());
+		if(std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end()) {
+			throw nemo::exception(NEMO_INVALID_INPUT, "a simulation appears more than once in the batch");
+		}
 		std::vector<Simulation*> ss(n);
 		for(size_t i = 0; i < n; ++i) {
 			requireSim(sims[i]);

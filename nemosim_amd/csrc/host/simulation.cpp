@@ -69,7 +69,9 @@ void
 DevBuf::ensure(size_t b)
 {
 	if(b > bytes) {
-		alloc(b);
+		/* geometric growth: buffers that follow a run's size are reallocated
+		 * O(log) times, not once per longer run */
+		alloc(std::max(b, bytes * 2));
 	}
 }
 
@@ -614,6 +616,7 @@ Simulation::Simulation(const NetworkImpl& net, const ConfigurationImpl& conf) :
 		loadPlugins(net);
 	}
 	hipCheck(hipHostMalloc(reinterpret_cast<void**>(&h_step), std::max<size_t>(m_words, 1) * 8), "hipHostMalloc");
+	ensureRaster(8);
 	hipCheck(hipStreamSynchronize(s), "hipStreamSynchronize");
 	devLock.unlock();
 
@@ -807,6 +810,7 @@ Simulation::~Simulation()
 	if(m_ev1) (void) hipEventDestroy(static_cast<hipEvent_t>(m_ev1));
 	if(h_ctl) (void) hipHostFree(h_ctl);
 	if(h_step) (void) hipHostFree(h_step);
+	if(h_sched) (void) hipHostFree(h_sched);
 	for(PluginGroup& pg : m_plugins) {
 		if(pg.handle) dlclose(pg.handle);
 	}
@@ -876,21 +880,33 @@ Simulation::uploadSchedule(unsigned nsteps, const nemo_schedule_t* s)
 	std::vector<uint32_t> fsPtr, fsIdx, isPtr, isIdx;
 	std::vector<float> isVal;
 	if(!translateSchedule(nsteps, s, fsPtr, fsIdx, isPtr, isIdx, isVal)) return;
-	d_fsPtr.ensure(fsPtr.size() * 4);
-	d_isPtr.ensure(isPtr.size() * 4);
-	d_fsIdx.ensure(std::max<size_t>(fsIdx.size(), 1) * 4);
-	d_isIdx.ensure(std::max<size_t>(isIdx.size(), 1) * 4);
-	d_isVal.ensure(std::max<size_t>(isVal.size(), 1) * 4);
-	hipStream_t st = static_cast<hipStream_t>(m_stream);
-	hipCheck(hipMemcpyAsync(d_fsPtr.ptr, fsPtr.data(), fsPtr.size() * 4, hipMemcpyHostToDevice, st), "H2D");
-	hipCheck(hipMemcpyAsync(d_isPtr.ptr, isPtr.data(), isPtr.size() * 4, hipMemcpyHostToDevice, st), "H2D");
-	if(!fsIdx.empty()) hipCheck(hipMemcpyAsync(d_fsIdx.ptr, fsIdx.data(), fsIdx.size() * 4, hipMemcpyHostToDevice, st), "H2D");
-	if(!isIdx.empty()) {
-		hipCheck(hipMemcpyAsync(d_isIdx.ptr, isIdx.data(), isIdx.size() * 4, hipMemcpyHostToDevice, st), "H2D");
-		hipCheck(hipMemcpyAsync(d_isVal.ptr, isVal.data(), isVal.size() * 4, hipMemcpyHostToDevice, st), "H2D");
+	/* one pinned staging block, one copy, no wait: the launch that reads the
+	 * schedule follows on the same stream and ends with a synchronisation,
+	 * before this staging block is written again */
+	const size_t words = fsPtr.size() + isPtr.size() + fsIdx.size() + 2 * isIdx.size();
+	if(words * 4 > m_schedHostBytes) {
+		const size_t bytes = std::max(words * 4, size_t(2) * m_schedHostBytes + 4096);
+		if(h_sched) (void) hipHostFree(h_sched);
+		h_sched = nullptr;
+		m_schedHostBytes = 0;
+		hipCheck(hipHostMalloc(reinterpret_cast<void**>(&h_sched), bytes), "hipHostMalloc");
+		m_schedHostBytes = bytes;
 	}
-	/* the host vectors go out of scope: wait for the copies */
-	hipCheck(hipStreamSynchronize(st), "hipStreamSynchronize");
+	uint32_t* w = h_sched;
+	w = std::copy(fsPtr.begin(), fsPtr.end(), w);
+	w = std::copy(isPtr.begin(), isPtr.end(), w);
+	w = std::copy(fsIdx.begin(), fsIdx.end(), w);
+	w = std::copy(isIdx.begin(), isIdx.end(), w);
+	if(!isVal.empty()) std::memcpy(w, isVal.data(), isVal.size() * 4);
+	d_sched.ensure(words * 4);
+	hipCheck(hipMemcpyAsync(d_sched.ptr, h_sched, words * 4, hipMemcpyHostToDevice,
+				static_cast<hipStream_t>(m_stream)), "H2D");
+	const uint32_t* base = d_sched.as<uint32_t>();
+	m_sFsPtr = base;
+	m_sIsPtr = m_sFsPtr + fsPtr.size();
+	m_sFsIdx = m_sIsPtr + isPtr.size();
+	m_sIsIdx = m_sFsIdx + fsIdx.size();
+	m_sIsVal = reinterpret_cast<const float*>(m_sIsIdx + isIdx.size());
 	m_fsPtrHost.swap(fsPtr);
 	m_isPtrHost.swap(isPtr);
 	m_hasSchedule = true;
@@ -930,8 +946,23 @@ NetDev
 Simulation::prepareLaunch(unsigned nsteps)
 {
 	hipCheck(hipSetDevice(m_device), "hipSetDevice");
-	d_raster.ensure(std::max<size_t>(size_t(nsteps) * m_words, 1) * 8);
+	ensureRaster(size_t(nsteps) * m_words * 8);
 	return descriptor(nsteps);
+}
+
+/* The raster buffer holds at least `bytes`. It starts at 1024 cycles (at
+ * most 32 MB) and doubles when a run needs more, so runs of varying length
+ * do not reallocate it (hipMalloc and hipFree synchronise with the device
+ * and cost tens of microseconds). */
+void
+Simulation::ensureRaster(size_t bytes)
+{
+	bytes = std::max<size_t>(bytes, 8);
+	if(bytes <= d_raster.bytes) return;
+	const size_t perCycle = std::max<size_t>(size_t(m_words) * 8, 8);
+	size_t cap = std::max<size_t>(d_raster.bytes * 2, perCycle * std::max<size_t>(1, std::min<size_t>(1024,
+					(size_t(32) << 20) / perCycle)));
+	d_raster.alloc(std::max(cap, bytes));
 }
 
 /* The launch descriptor: network arrays, per-launch fields, the schedule
@@ -977,11 +1008,11 @@ Simulation::descriptor(unsigned nsteps) const
 	nd.stdp = d_stdp.ptr;
 	nd.srow = d_srow.as<uint64_t>();
 	if(m_hasSchedule) {
-		nd.fs_ptr = d_fsPtr.as<uint32_t>();
-		nd.fs_idx = d_fsIdx.as<uint32_t>();
-		nd.is_ptr = d_isPtr.as<uint32_t>();
-		nd.is_idx = d_isIdx.as<uint32_t>();
-		nd.is_val = d_isVal.as<float>();
+		nd.fs_ptr = m_sFsPtr;
+		nd.fs_idx = m_sFsIdx;
+		nd.is_ptr = m_sIsPtr;
+		nd.is_idx = m_sIsIdx;
+		nd.is_val = m_sIsVal;
 	}
 	nd.raster = d_raster.as<uint64_t>();
 	return nd;
@@ -1049,7 +1080,7 @@ Simulation::launchPlugins(unsigned nsteps, unsigned long long* events)
 	std::lock_guard<std::recursive_mutex> devLock(deviceMutex());
 	hipStream_t st = static_cast<hipStream_t>(m_stream);
 	hipCheck(hipSetDevice(m_device), "hipSetDevice");
-	d_raster.ensure(std::max<size_t>(size_t(nsteps) * m_words32, 2) * 4);
+	ensureRaster(size_t(nsteps) * m_words32 * 4);
 	hipCheck(nemo_dev_fill_u32(d_ctl.as<uint32_t>(), 0, 16, st), "fill");
 	PluginStepDev s;
 	std::memset(&s, 0, sizeof(s));
@@ -1074,11 +1105,11 @@ Simulation::launchPlugins(unsigned nsteps, unsigned long long* events)
 	s.rcm_acc = d_rcmAcc.as<int32_t>();
 	s.stdp = d_stdp.ptr;
 	if(m_hasSchedule) {
-		s.fs_ptr = d_fsPtr.as<uint32_t>();
-		s.fs_idx = d_fsIdx.as<uint32_t>();
-		s.is_ptr = d_isPtr.as<uint32_t>();
-		s.is_idx = d_isIdx.as<uint32_t>();
-		s.is_val = d_isVal.as<float>();
+		s.fs_ptr = m_sFsPtr;
+		s.fs_idx = m_sFsIdx;
+		s.is_ptr = m_sIsPtr;
+		s.is_idx = m_sIsIdx;
+		s.is_val = m_sIsVal;
 	}
 	hipEvent_t e0 = static_cast<hipEvent_t>(m_ev0), e1 = static_cast<hipEvent_t>(m_ev1);
 	hipCheck(hipEventRecord(e0, st), "hipEventRecord");
@@ -1260,7 +1291,7 @@ Simulation::readRaster(uint64_t words[], size_t cap, size_t* wordsPerStep)
 	size_t nw = size_t(m_lastRunSteps) * m_words;
 	if(wordsPerStep) *wordsPerStep = m_words;
 	if(words) {
-			if(cap < nw) throw exception(NEMO_BUFFER_OVERFLOW, "raster buffer too small");
+		if(cap < nw) throw exception(NEMO_BUFFER_OVERFLOW, "raster buffer too small");
 		if(nw) hipCheck(hipMemcpy(words, d_raster.ptr, nw * 8, hipMemcpyDeviceToHost), "D2H");
 	}
 }

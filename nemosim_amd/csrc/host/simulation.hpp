@@ -185,7 +185,16 @@ class Simulation
 		DevBuf d_off, d_syn;
 		DevBuf d_fl, d_fc, d_rfg, d_srow, d_ctl;       /* ctl: bar, status, events */
 		DevBuf d_rcmStart, d_rcmSd, d_rcmFwd, d_rcmAcc, d_stdp;
-		DevBuf d_fsPtr, d_fsIdx, d_isPtr, d_isIdx, d_isVal;
+		/* the stimulus schedule of the next launch: one device block, staged
+		 * through pinned host memory, and its parts */
+		DevBuf d_sched;
+		uint32_t* h_sched = nullptr;
+		size_t m_schedHostBytes = 0;
+		const uint32_t* m_sFsPtr = nullptr;
+		const uint32_t* m_sFsIdx = nullptr;
+		const uint32_t* m_sIsPtr = nullptr;
+		const uint32_t* m_sIsIdx = nullptr;
+		const float* m_sIsVal = nullptr;
 		DevBuf d_raster;
 		DevBuf d_work;
 		size_t m_deviceBytes = 0;
@@ -206,6 +215,7 @@ class Simulation
 		/* the step kernel's launch: descriptor by value, no copies before it */
 		void launchFused(unsigned nsteps, bool stepOut);
 		void launchDescriptor(const NetDev& nd, bool stepOut);
+		void ensureRaster(size_t bytes);
 		NetDev descriptor(unsigned nsteps) const;
 		bool translateSchedule(unsigned nsteps, const nemo_schedule_t* s, std::vector<uint32_t>& fsPtr,
 				std::vector<uint32_t>& fsIdx, std::vector<uint32_t>& isPtr, std::vector<uint32_t>& isIdx,

@@ -149,3 +149,18 @@ def test_delivery_lanes_validation(nemo_mod):
         with pytest.raises(nemo_mod.NemoError) as e:
             conf.set_delivery_lanes(bad)
         assert e.value.status == 6   # NEMO_INVALID_INPUT
+
+
+def test_batch_rejects_a_simulation_listed_twice(nemo_mod):
+    """nemo_hip_run_batch refuses a batch naming one simulation twice before
+    touching it (the two copies would share state and exchange buffers in
+    one launch); checked on the host with a stand-in handle that is never
+    dereferenced."""
+    from nemosim_amd import _lib
+    lib = _lib.load()
+    token = ctypes.create_string_buffer(64)
+    handles = (ctypes.c_void_p * 2)(ctypes.addressof(token), ctypes.addressof(token))
+    steps = (ctypes.c_uint * 2)(10, 10)
+    fn = lib.nemo_hip_run_batch
+    assert fn(handles, 2, steps, None) == 6
+    assert b"more than once" in lib.nemo_strerror()
