@@ -131,6 +131,23 @@ typedef struct nemo_schedule {
 nemo_status_t nemo_hip_run(nemo_simulation_t sim, unsigned nsteps,
 		const nemo_schedule_t* schedule, unsigned long long* events);
 
+/* No counterpart in NeMo: nemo_hip_run for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). It allocates nothing: the raster of the
+ * run goes to the caller's device buffer raster_dev (nsteps x words_per_step
+ * 64-bit words, words_per_step from nemo_hip_read_raster; layout as
+ * there), and the stimulus schedule is staged in a fixed per-device scratch
+ * area of NEMO_HIP_STATIC_SCRATCH_BYTES: a schedule that needs more 4-byte
+ * words than fit there -- 2 (nsteps + 1) + forced firings + 2 x current
+ * assignments -- is refused with NEMO_INVALID_INPUT before anything runs. Every
+ * call on a device shares that device's scratch area, so calls that overlap
+ * on one device race. Fused step kernel only (NEMO_API_UNSUPPORTED
+ * otherwise). nemo_hip_read_fired / nemo_hip_read_raster refuse to read
+ * such a run; events (may be NULL) as nemo_hip_run. */
+#define NEMO_HIP_STATIC_SCRATCH_BYTES (4u << 20)
+nemo_status_t nemo_hip_run_static(nemo_simulation_t sim, unsigned nsteps,
+		const nemo_schedule_t* schedule, uint64_t* raster_dev, unsigned long long* events);
+
 /* Fired neurons of the last run: counts[k] for each of its cycles, and the
  * user indices of all fired neurons back to back (cycle order, within a
  * cycle in the order nemo_step reports them). ids may be NULL; cap is its

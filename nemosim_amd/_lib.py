@@ -136,6 +136,8 @@ SIGNATURES = {
     "nemo_set_device_construction": (ctypes.c_int, [c_void_p, ctypes.c_int]),
     "nemo_set_step_mode": (ctypes.c_int, [c_void_p, ctypes.c_int]),
     "nemo_hip_run": (ctypes.c_int, [c_void_p, ctypes.c_uint, ctypes.POINTER(ScheduleT), c_ull_p]),
+    "nemo_hip_run_static": (ctypes.c_int, [c_void_p, ctypes.c_uint, ctypes.POINTER(ScheduleT), c_void_p,
+                                           c_ull_p]),
     "nemo_hip_read_fired": (ctypes.c_int, [c_void_p, c_uint_p, c_uint_p, ctypes.c_size_t, c_size_p]),
     "nemo_hip_read_raster": (ctypes.c_int, [c_void_p, c_u64_p, ctypes.c_size_t, c_size_p]),
     "nemo_hip_run_batch": (ctypes.c_int, [ctypes.POINTER(c_void_p), ctypes.c_size_t, c_uint_p, c_ull_p]),

@@ -129,6 +129,12 @@ int nemo_dev_batch_graph_create(const NetDev* nets_dev, uint32_t nnets, const Wo
 		void** exec_out);
 int nemo_dev_graph_launch(void* exec, void* stream);
 int nemo_dev_graph_destroy(void* exec);
+/* The allocation-free run's scratch array: 4-byte words, one array per
+ * device. nemo_dev_static_scratch gives its device address on the current
+ * device (looked up once per device with hipGetSymbolAddress). The caller
+ * holds the library's device lock. */
+#define NEMO_STATIC_SCRATCH_WORDS (1u << 20)
+int nemo_dev_static_scratch(void** ptr);
 int nemo_dev_apply_stdp(uint32_t* syn, const uint32_t* rcm_fwd, int32_t* rcm_acc, size_t nr,
 		const void* stdp_dev, int32_t fx_reward, void* stream);
 int nemo_dev_fill_u32(uint32_t* p, uint32_t value, size_t n, void* stream);

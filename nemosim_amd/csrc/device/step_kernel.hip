@@ -1395,6 +1395,14 @@ __global__ void nemo_nrand_seq_kernel(const uint32_t* state4, float* out, size_t
 	}
 }
 
+/* Scratch memory of the allocation-free run (nemo_hip_run_static): the
+ * stimulus schedule of a run is staged here instead of in buffers the
+ * library allocates. Each device holds its own copy of the array for the
+ * life of the process; host code reaches it only through the address
+ * hipGetSymbolAddress returns on that device (the array's name is not a
+ * device address on the host). Calls on one device share it. */
+static __device__ __align__(256) uint32_t scratch[NEMO_STATIC_SCRATCH_WORDS];
+
 static unsigned grid_for(size_t n, unsigned block)
 {
 	size_t g = (n + block - 1) / block;
@@ -1528,6 +1536,23 @@ int nemo_dev_step_max_blocks(uint32_t blockDim, uint32_t ldsBytes, uint32_t vari
 		perCU = std::min(perCU, int(uint32_t(ldsPerCU) / alloc));
 	}
 	*maxBlocks = uint32_t(perCU) * uint32_t(cus);
+	return 0;
+}
+
+int nemo_dev_static_scratch(void** ptr)
+{
+	static void* addr[64] = {};
+	int dev = 0;
+	hipError_t e = hipGetDevice(&dev);
+	if(e != hipSuccess) return int(e);
+	if(dev < 0 || dev >= 64) return int(hipErrorInvalidDevice);
+	if(!addr[dev]) {
+		void* p = nullptr;
+		e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+		if(e != hipSuccess) return int(e);
+		addr[dev] = p;
+	}
+	*ptr = addr[dev];
 	return 0;
 }
 

@@ -316,6 +316,13 @@ nemo_hip_run(nemo_simulation_t s, unsigned nsteps, const nemo_schedule_t* sched,
 }
 
 nemo_status_t
+nemo_hip_run_static(nemo_simulation_t s, unsigned nsteps, const nemo_schedule_t* sched, uint64_t* raster_dev,
+		unsigned long long* events)
+{
+	RETURN_CALL(requireSim(s); s->sim->runStatic(nsteps, sched, raster_dev, events));
+}
+
+nemo_status_t
 nemo_hip_read_fired(nemo_simulation_t s, unsigned counts[], unsigned ids[], size_t cap, size_t* total)
 {
 	RETURN_CALL(requireSim(s); s->sim->readFired(counts, ids, cap, total));

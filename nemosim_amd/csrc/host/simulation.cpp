@@ -947,6 +947,7 @@ Simulation::prepareLaunch(unsigned nsteps)
 {
 	hipCheck(hipSetDevice(m_device), "hipSetDevice");
 	ensureRaster(size_t(nsteps) * m_words * 8);
+	m_rasterExternal = false;
 	return descriptor(nsteps);
 }
 
@@ -1082,6 +1083,7 @@ Simulation::launchPlugins(unsigned nsteps, unsigned long long* events)
 	hipCheck(hipSetDevice(m_device), "hipSetDevice");
 	ensureRaster(size_t(nsteps) * m_words32 * 4);
 	hipCheck(nemo_dev_fill_u32(d_ctl.as<uint32_t>(), 0, 16, st), "fill");
+	m_rasterExternal = false;
 	PluginStepDev s;
 	std::memset(&s, 0, sizeof(s));
 	s.n = m_n;
@@ -1207,6 +1209,73 @@ Simulation::launchDescriptor(const NetDev& nd, bool stepOut)
 	m_kernelMs = ms;
 }
 
+static_assert(size_t(NEMO_STATIC_SCRATCH_WORDS) * 4u == size_t(NEMO_HIP_STATIC_SCRATCH_BYTES),
+		"the scratch array and its published size agree");
+
+/* nemo_hip_run without any device allocation: the stimulus schedule is
+ * staged in the per-device scratch array, the raster goes to the caller's
+ * device buffer, and everything else the kernel touches was allocated when
+ * the simulation was built. */
+void
+Simulation::runStatic(unsigned nsteps, const nemo_schedule_t* schedule, uint64_t* rasterDev,
+		unsigned long long* events)
+{
+	std::lock_guard<std::recursive_mutex> devLock(deviceMutex());
+	if(events) *events = 0;
+	if(!m_fused) {
+		throw exception(NEMO_API_UNSUPPORTED,
+				"the allocation-free run needs the fused step kernel; this simulation runs on the plugin step");
+	}
+	const bool has = translateSchedule(nsteps, schedule, m_stFsPtr, m_stFsIdx, m_stIsPtr, m_stIsIdx, m_stIsVal);
+	const size_t words = has ? m_stFsPtr.size() + m_stIsPtr.size() + m_stFsIdx.size() + 2 * m_stIsIdx.size() : 0;
+	if(words > NEMO_STATIC_SCRATCH_WORDS) {
+		throw exception(NEMO_INVALID_INPUT, fmt("stimulus schedule of ", words,
+					" words is larger than the allocation-free run's scratch of ", NEMO_STATIC_SCRATCH_WORDS,
+					" 4-byte words; use nemo_hip_run"));
+	}
+	if(nsteps == 0 || m_P == 0 || m_n == 0) {
+		m_cycle += nsteps;
+		m_timerCycles += nsteps;
+		m_lastRunSteps = nsteps;
+		m_rasterExternal = true;
+		return;
+	}
+	if(!rasterDev) throw exception(NEMO_INVALID_INPUT, "null raster buffer");
+	hipCheck(hipSetDevice(m_device), "hipSetDevice");
+	void* scr = nullptr;
+	hipCheck(nemo_dev_static_scratch(&scr), "allocation-free run: scratch address");
+	m_hasSchedule = false;
+	NetDev nd = descriptor(nsteps);
+	nd.raster = rasterDev;
+	nd.fs_ptr = nullptr;
+	nd.fs_idx = nullptr;
+	nd.is_ptr = nullptr;
+	nd.is_idx = nullptr;
+	nd.is_val = nullptr;
+	if(has) {
+		/* one staging array, one copy: [fired ptr][current ptr][fired idx]
+		 * [current idx][current value bits] */
+		m_stage.resize(words);
+		uint32_t* w = m_stage.data();
+		w = std::copy(m_stFsPtr.begin(), m_stFsPtr.end(), w);
+		w = std::copy(m_stIsPtr.begin(), m_stIsPtr.end(), w);
+		w = std::copy(m_stFsIdx.begin(), m_stFsIdx.end(), w);
+		w = std::copy(m_stIsIdx.begin(), m_stIsIdx.end(), w);
+		if(!m_stIsVal.empty()) std::memcpy(w, m_stIsVal.data(), m_stIsVal.size() * 4);
+		hipCheck(hipMemcpyAsync(scr, m_stage.data(), words * 4, hipMemcpyHostToDevice,
+					static_cast<hipStream_t>(m_stream)), "H2D");
+		const uint32_t* base = static_cast<const uint32_t*>(scr);
+		nd.fs_ptr = base;
+		nd.is_ptr = base + m_stFsPtr.size();
+		nd.fs_idx = nd.is_ptr + m_stIsPtr.size();
+		nd.is_idx = nd.fs_idx + m_stFsIdx.size();
+		nd.is_val = reinterpret_cast<const float*>(nd.is_idx + m_stIsIdx.size());
+	}
+	launchDescriptor(nd, false);
+	readControl(nsteps, events);
+	m_rasterExternal = true;
+}
+
 void
 Simulation::decodeStep(const uint64_t* words, std::vector<unsigned>& out) const
 {
@@ -1263,6 +1332,7 @@ void
 Simulation::readFired(unsigned counts[], unsigned ids[], size_t cap, size_t* total)
 {
 	std::lock_guard<std::recursive_mutex> devLock(deviceMutex());
+	requireOwnRaster();
 	size_t nw = size_t(m_lastRunSteps) * m_words;
 	m_rasterHost.resize(nw);
 	if(nw) {
@@ -1291,8 +1361,18 @@ Simulation::readRaster(uint64_t words[], size_t cap, size_t* wordsPerStep)
 	size_t nw = size_t(m_lastRunSteps) * m_words;
 	if(wordsPerStep) *wordsPerStep = m_words;
 	if(words) {
+		requireOwnRaster();
 		if(cap < nw) throw exception(NEMO_BUFFER_OVERFLOW, "raster buffer too small");
 		if(nw) hipCheck(hipMemcpy(words, d_raster.ptr, nw * 8, hipMemcpyDeviceToHost), "D2H");
+	}
+}
+
+void
+Simulation::requireOwnRaster() const
+{
+	if(m_rasterExternal) {
+		throw exception(NEMO_INVALID_INPUT,
+				"the last run wrote its raster to the caller's buffer (nemo_hip_run_static)");
 	}
 }
 

@@ -74,7 +74,9 @@ class Simulation
 
 		/* multi-cycle runs */
 		void run(unsigned nsteps, const nemo_schedule_t* schedule, unsigned long long* events);
-
+		/* run() with no device allocation (nemo_hip_run_static) */
+		void runStatic(unsigned nsteps, const nemo_schedule_t* schedule, uint64_t* rasterDev,
+				unsigned long long* events);
 		void readFired(unsigned counts[], unsigned ids[], size_t cap, size_t* total);
 		void readRaster(uint64_t words[], size_t cap, size_t* wordsPerStep);
 		void info(nemo_hip_info_t* out) const;
@@ -220,7 +222,11 @@ class Simulation
 		bool translateSchedule(unsigned nsteps, const nemo_schedule_t* s, std::vector<uint32_t>& fsPtr,
 				std::vector<uint32_t>& fsIdx, std::vector<uint32_t>& isPtr, std::vector<uint32_t>& isIdx,
 				std::vector<float>& isVal) const;
-
+		/* the allocation-free run: schedule staging and the raster's place */
+		std::vector<uint32_t> m_stFsPtr, m_stFsIdx, m_stIsPtr, m_stIsIdx, m_stage;
+		std::vector<float> m_stIsVal;
+		bool m_rasterExternal = false;   /* the last run's raster is in the caller's buffer */
+		void requireOwnRaster() const;
 		void readControl(unsigned nsteps, unsigned long long* events);
 
 		/* schedule of the current launch (host side, sim-local indices) */
