@@ -92,14 +92,16 @@ NEMO_HD inline float fx20_to_float(int32_t v)
 	return float(v) / 1048576.0f;
 }
 
-/* Izhikevich update of one neuron, src/nemo/cpu/plugins/Izhikevich.cpp:53-94.
+/* Izhikevich update of one neuron, src/nemo/cpu/plugins/Izhikevich.cpp:53-94,
+ * with the Gaussian input sigma * nrand() already drawn (`noise`, added only
+ * when sigma != 0, after E + I + external as the reference adds it).
  * p = {a, b, c, d, sigma}; s = {u, v}. Returns the fired flag. */
-NEMO_HD inline bool izhikevich(const float* p, float* s, Rng& rng,
+NEMO_HD inline bool izhikevich_drawn(const float* p, float* s, float noise,
 		float currentE, float currentI, float currentExt, bool fstim)
 {
 	float I = currentE + currentI + currentExt;
 	if(p[IZ_SIGMA] != 0.0f) {
-		I += p[IZ_SIGMA] * nrand(rng);
+		I += noise;
 	}
 	bool fired = false;
 	float u = s[IZ_U];
@@ -122,6 +124,14 @@ NEMO_HD inline bool izhikevich(const float* p, float* s, Rng& rng,
 	s[IZ_U] = u;
 	s[IZ_V] = v;
 	return fired;
+}
+
+/* Izhikevich update drawing its Gaussian input from the neuron's stream */
+NEMO_HD inline bool izhikevich(const float* p, float* s, Rng& rng,
+		float currentE, float currentI, float currentExt, bool fstim)
+{
+	const float noise = (p[IZ_SIGMA] != 0.0f) ? p[IZ_SIGMA] * nrand(rng) : 0.0f;
+	return izhikevich_drawn(p, s, noise, currentE, currentI, currentExt, fstim);
 }
 
 /* IF_curr_exp update of one neuron, src/nemo/cpu/plugins/IF_curr_exp.cpp:177-222;

@@ -121,7 +121,7 @@ int nemo_dev_step_launch(const NetDev* nets_dev, const WorkItem* work_dev, uint3
 int nemo_dev_step_launch_one(const NetDev* nd_host, const WorkItem* work_dev, uint32_t nblocks,
 		uint32_t blockDim, uint32_t ldsBytes, uint32_t variant, void* stream);
 int nemo_dev_step_max_blocks(uint32_t blockDim, uint32_t ldsBytes, uint32_t variant, uint32_t* maxBlocks);
-/* Batched step as an instantiated HIP graph (clear -> cooperative step ->
+/* Batched step as an instantiated HIP graph (clear -> persistent step ->
  * collect); collect_dev receives [nnets][8] words: the first 8 words of each
  * network's control block. The graph reads nets_dev/work_dev at replay time. */
 int nemo_dev_batch_graph_create(const NetDev* nets_dev, uint32_t nnets, const WorkItem* work_dev,

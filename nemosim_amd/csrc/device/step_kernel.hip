@@ -1,9 +1,11 @@
 /*
  * step_kernel.hip -- the simulation step on gfx950.
  *
- * One persistent, cooperatively launched kernel runs many 1 ms cycles of one
- * or more independent networks. Each workgroup owns one partition of T
- * target neurons of one network:
+ * One persistent kernel runs many 1 ms cycles of one or more independent
+ * networks. Each workgroup owns one partition of T target neurons of one
+ * network, and every workgroup of a launch is resident at once (the host
+ * sizes launches by the device's measured co-residency; the workgroups of
+ * a network exchange fired lists every cycle):
  *
  *   - the partition's neuron parameters, state, RNG and 64-cycle firing
  *     history stay in registers for the whole launch (one neuron per thread);
@@ -846,17 +848,32 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 	/* deferred delivery: this wave's far ranges [a, b) (delays >= 2) of the
 	 * previous cycle's spikes, one source per lane, kn sources (wave-uniform) */
 	uint32_t ka0 = 0, kb0 = 0, ka1 = 0, kb1 = 0, kn = 0;
+	/* Izhikevich-only networks draw each neuron's Gaussian input of the next
+	 * cycle ahead, while the wave waits for memory (the first cycle's here):
+	 * sigma * nrand() does not depend on the cycle's input, and every cycle
+	 * takes exactly one draw, so the stream is the same. The last cycle of a
+	 * launch draws nothing ahead, so the saved RNG state is exact. */
+	float noise = 0.0f;
+	if(IZO && active && nd.nsteps > 0u && prm[IZ_SIGMA] != 0.0f) {
+		noise = prm[IZ_SIGMA] * nrand(rng);
+	}
+	/* ring slot t mod D and fired-list buffer t mod flRing, kept as counters */
+	uint32_t slotK = uint32_t(nd.cycle0 % D);
+	uint32_t flbK = uint32_t(nd.cycle0 % nd.flRing);
 	__syncthreads();
 
 	for(uint32_t k = 0; k < nd.nsteps; ++k) {
 		const uint64_t t = nd.cycle0 + k;
-		const uint32_t slot = uint32_t(t % D);
+		const uint32_t slot = slotK;
 		const uint32_t par = uint32_t(t & 1u);
 		/* fired-list and count buffer of this cycle: a ring of flRing
 		 * cycles (at least 2; 3 when deferred delivery reads the previous
 		 * cycle's lists while the next are written; postLen+2 with STDP,
 		 * which reads the lists of cycle t-postLen) */
-		const uint32_t flb = uint32_t(t % nd.flRing);
+		const uint32_t flb = flbK;
+		slotK = (slotK + 1u == D) ? 0u : slotK + 1u;
+		flbK = (flbK + 1u == nd.flRing) ? 0u : flbK + 1u;
+		const bool drawn = !(IZO && active && k + 1u < nd.nsteps && prm[IZ_SIGMA] != 0.0f);
 
 		/* ---- stimulus of this step (sim-local indices, deduplicated by the host) */
 		bool has_stim = false;
@@ -905,7 +922,7 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 				fs = ((fst[tid >> 5] >> (tid & 31u)) & 1u) != 0u;
 			}
 			if(IZO) {
-				fired = izhikevich(prm, st, rng, fx20_to_float(e), fx20_to_float(i), cx, fs);
+				fired = izhikevich_drawn(prm, st, noise, fx20_to_float(e), fx20_to_float(i), cx, fs);
 			} else {
 				fired = update_neuron(model, prm, st, rng, fx20_to_float(e), fx20_to_float(i), cx, fs);
 			}
@@ -970,6 +987,12 @@ __device__ void run_partition(const NetDev& nd, uint32_t p)
 			if(tid == 0) {
 				st_agent(glb(nd.fc) + flb * P + p, (uint64_t(uint32_t(t)) << 32) | uint64_t(total));
 			}
+		}
+
+		/* the next cycle's Gaussian input, drawn while the memory-bound
+		 * phases below keep the SIMDs' other waves busy */
+		if(!drawn) {
+			noise = prm[IZ_SIGMA] * nrand(rng);
 		}
 
 		/* ---- deferred records (delays >= 2) of the previous cycle's spikes,
@@ -1420,8 +1443,7 @@ int nemo_dev_step_launch(const NetDev* nets_dev, const WorkItem* work_dev, uint3
 	hipError_t e = ensure_lds_limit(kern, ldsBytes);
 	if(e != hipSuccess) return int(e);
 	void* args[] = { const_cast<NetDev**>(&nets_dev), const_cast<WorkItem**>(&work_dev) };
-	e = hipLaunchCooperativeKernel(kern,
-			dim3(nblocks), dim3(blockDim), args, ldsBytes, static_cast<hipStream_t>(stream));
+	e = hipLaunchKernel(kern, dim3(nblocks), dim3(blockDim), args, ldsBytes, static_cast<hipStream_t>(stream));
 	return int(e);
 }
 
@@ -1432,12 +1454,11 @@ int nemo_dev_step_launch_one(const NetDev* nd_host, const WorkItem* work_dev, ui
 	hipError_t e = ensure_lds_limit(kern, ldsBytes);
 	if(e != hipSuccess) return int(e);
 	void* args[] = { const_cast<NetDev*>(nd_host), const_cast<WorkItem**>(&work_dev) };
-	e = hipLaunchCooperativeKernel(kern,
-			dim3(nblocks), dim3(blockDim), args, ldsBytes, static_cast<hipStream_t>(stream));
+	e = hipLaunchKernel(kern, dim3(nblocks), dim3(blockDim), args, ldsBytes, static_cast<hipStream_t>(stream));
 	return int(e);
 }
 
-/* The batched step as a HIP graph: clear -> step (cooperative) -> collect.
+/* The batched step as a HIP graph: clear -> step -> collect.
  * Everything a replay needs is read from the device tables nets_dev and
  * work_dev, so one instantiated graph serves every launch of the same
  * work table; the host rewrites nets_dev before each replay. */
@@ -1473,12 +1494,6 @@ int nemo_dev_batch_graph_create(const NetDev* nets_dev, uint32_t nnets, const Wo
 		kp.sharedMemBytes = ldsBytes;
 		kp.kernelParams = stepArgs;
 		e = hipGraphAddKernelNode(&nStep, g, &nClear, 1, &kp);
-	}
-	if(e == hipSuccess) {
-		/* all workgroups must be co-resident: they exchange fired lists */
-		hipKernelNodeAttrValue coop = {};
-		coop.cooperative = 1;
-		e = hipGraphKernelNodeSetAttribute(nStep, hipKernelNodeAttributeCooperative, &coop);
 	}
 	void* collectArgs[] = { const_cast<NetDev**>(&nets_dev), &nnets, &collect_dev };
 	if(e == hipSuccess) {
@@ -1527,7 +1542,12 @@ int nemo_dev_step_max_blocks(uint32_t blockDim, uint32_t ldsBytes, uint32_t vari
 	 * query when LDS is the limit: each workgroup takes its LDS rounded up to
 	 * 512 bytes plus 512 more (32536 B: 4 per CU, not 5; 16000 B: 9, not 10;
 	 * 8000 B: 18, not 20). A persistent launch needs every workgroup
-	 * resident, so the count follows the measurement. */
+	 * resident, so the count follows the measurement. This bound is what
+	 * keeps launches co-resident: they are ordinary launches (a cooperative
+	 * launch checks against the occupancy query, which over-reports here,
+	 * and costs about 15 us more per launch), and every wait for another
+	 * workgroup is bounded, so a launch that did not fit would end with an
+	 * error, not hang. */
 	int ldsPerCU = 0;
 	e = hipDeviceGetAttribute(&ldsPerCU, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev);
 	if(e != hipSuccess) return int(e);

@@ -1,7 +1,7 @@
 /*
  * batch.cpp -- the batched form of the step (nemo_hip_run_batch).
  *
- * A pack of networks becomes one cooperative launch whose work table maps
+ * A pack of networks becomes one persistent launch whose work table maps
  * each workgroup to (network, partition), XCD-aware over the whole pack.
  * The launch is a HIP graph of three kernels -- clear the networks' status
  * words and event counters, the step kernel, gather status and events into

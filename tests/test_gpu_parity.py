@@ -640,3 +640,33 @@ def test_c3_60_epochs(nemo_mod, oracle_mod, progress):
         orc.apply_stdp(1.0)
         progress("C3 second %d identical, %d spikes" % (epoch, int(rc.sum())))
     compare_weights(sim, orc, spec, 200000, seed=5)
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("n,m,dmax,stdp_on", [(300000, 100, 20, False), (100000, 100, 64, False),
+                                              (80000, 100, 64, True)])
+def test_beyond_the_fused_range(nemo_mod, oracle_mod, progress, n, m, dmax, stdp_on):
+    """Networks the fused kernel cannot hold -- more partitions than can be
+    co-resident (300k neurons at delay 20, 100k at delay 64) or an LDS ring
+    too deep for enough partitions (80k at delay 64, with STDP) -- run on the
+    plugin step in the automatic mode: 200 cycles (with applyStdp after 100)
+    against the oracle, full raster and a sample of the state and weights."""
+    spec = examples.izhikevich_80_20(n, m, seed=61, dmax=dmax, stdp=stdp_on)
+    stdp = None
+    if stdp_on:
+        pre, post = examples.stdp_window()
+        stdp = (pre, post, 0.0, 1.0, 0.0, -1.0)
+    sim = hip_sim(nemo_mod, spec, stdp=stdp)
+    info = sim.info()
+    assert info["fused"] == 0, info
+    orc = oracle_mod.from_spec(spec, stdp=stdp)
+    rc, _ = compare_run(sim, orc, 100)
+    if stdp_on:
+        sim.apply_stdp(1.0)
+        orc.apply_stdp(1.0)
+    compare_run(sim, orc, 100)
+    progress("%d x %d, delays up to %d%s: plugin step, 200 cycles identical (%d spikes in the first 100)"
+             % (n, m, dmax, ", STDP" if stdp_on else "", int(rc.sum())))
+    compare_state(sim, orc, spec.n, 2, sample=range(0, spec.n, 997))
+    if stdp_on:
+        compare_weights(sim, orc, spec, 3000, seed=7)
