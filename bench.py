@@ -265,9 +265,8 @@ def bench_plain(args, stdp=False):
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            # measured for the default workload only (profiles/ holds its counters)
-            "traffic": (traffic_from_profile(args.traffic_file)
-                        if (not stdp and n == 200000 and m == 1000 and L == 1000) else None),
+            # rocprof FETCH_SIZE + WRITE_SIZE per launch of this shape (profiles/), if measured
+            "traffic": traffic_from_profile(profile_for(stdp, n, m, L)),
             "algorithmic_bytes_per_launch": abytes / launches,
             "kernel_ms_per_launch": kernel_ms / launches,
         },
@@ -342,6 +341,17 @@ def bench_step(args):
         "config": {"workload": "nemo_step per cycle at 1k x 100 and 200k x 1000, STDP off"},
         "sizes": lines,
     }
+
+
+def profile_for(stdp, n, m, launch):
+    """The committed counter summary of this workload and launch length."""
+    if not stdp and n == 200000 and m == 1000:
+        name = "nemo_step_kernel_c2_launch%d_counters.txt" % launch
+    elif stdp and n == 50000 and m == 1000 and launch == 1000:
+        name = "nemo_step_kernel_c3_stdp_counters.txt"
+    else:
+        return None
+    return os.path.join(ROOT, "profiles", name)
 
 
 def traffic_from_profile(path):
@@ -475,7 +485,6 @@ def main():
     ap.add_argument("--no-parity", action="store_true", help="skip the oracle replay of the GPU run's cycles")
     ap.add_argument("--parity-seconds", type=float, default=60.0,
                     help="host time the oracle may spend replaying the GPU run's cycles")
-    ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "nemo_step_kernel_c2_counters.txt"))
     ap.add_argument("--batch-networks", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=2000)
     ap.add_argument("--serve-requests", type=int, default=200)
