@@ -429,6 +429,33 @@ def bench_serve(args):
                          tuning=autotune.load_table() if args.serve_tuned else None)
     log("serve: %d requests, budget %.0f GiB, chunks of %d cycles" % (len(reqs), args.serve_budget_gb, args.serve_chunk))
     stats = srv.serve(reqs, realtime=True)
+    # sustained load: latency and its parts by arrival order (tenths of the
+    # requests), queue depth over the run, offered against completed rate
+    arr = np.array([r.arrival_s for r in reqs])
+    adm = np.array([r.admitted_s for r in reqs])
+    blt = np.array([r.built_s for r in reqs])
+    fin = np.array([r.done_s for r in reqs])
+    tenths = np.array_split(np.arange(len(reqs)), 10)
+    by_arrival = [{"requests": [int(ix[0]), int(ix[-1])],
+                   "latency_s_mean": float(np.mean(fin[ix] - arr[ix])),
+                   "latency_s_p99": float(np.percentile(fin[ix] - arr[ix], 99)),
+                   "wait_s_mean": float(np.mean(adm[ix] - arr[ix])),
+                   "build_s_mean": float(np.mean(blt[ix] - adm[ix])),
+                   "run_s_mean": float(np.mean(fin[ix] - blt[ix]))} for ix in tenths if len(ix)]
+    d = np.array(srv.depth) if srv.depth else np.zeros((0, 4))
+    quarters = np.array_split(np.arange(len(d)), 4)
+    depth = {"max_waiting": int(d[:, 1].max()) if len(d) else 0,
+             "max_building": int(d[:, 2].max()) if len(d) else 0,
+             "max_resident": int(d[:, 3].max()) if len(d) else 0,
+             "mean_queued_by_quarter": [float(np.mean(d[q, 1] + d[q, 2])) for q in quarters if len(q)],
+             "mean_resident_by_quarter": [float(np.mean(d[q, 3])) for q in quarters if len(q)]}
+    offered = len(reqs) / float(arr[-1]) if len(reqs) else None
+    log("serve: offered %.1f req/s over %.1f s, completed %d in %.1f s" % (offered, arr[-1], stats["requests"], stats["wall_s"]))
+    for b in by_arrival:
+        log("  requests %4d-%4d: latency mean %.2f s (wait %.2f, build %.2f, run %.2f), p99 %.2f s"
+            % (b["requests"][0], b["requests"][1], b["latency_s_mean"], b["wait_s_mean"], b["build_s_mean"],
+               b["run_s_mean"], b["latency_s_p99"]))
+    log("  queue depth %s" % depth)
     return {
         "metric": "synaptic events/s (serving)",
         "value": stats["events_per_s"],
@@ -446,8 +473,11 @@ def bench_serve(args):
         "p50_latency_s": stats["p50_latency_s"],
         "p99_latency_s": stats["p99_latency_s"],
         "requests_per_s": stats["requests_per_s"],
+        "offered_requests_per_s": offered,
         "gpu_busy_s": stats["gpu_busy_s"],
         "wall_s": stats["wall_s"],
+        "latency_by_arrival": by_arrival,
+        "queue_depth": depth,
     }
 
 

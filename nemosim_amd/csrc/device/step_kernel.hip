@@ -1405,7 +1405,8 @@ __global__ void nemo_fill_u64_kernel(uint64_t* p, uint64_t v, size_t n)
 __global__ void nemo_eval_libm_kernel(int kind, const float* in, float* out, size_t n)
 {
 	for(size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
-		out[i] = (kind == 0) ? nemo_libm::logf_glibc(in[i]) : nemo_libm::sinf_glibc(in[i]);
+		out[i] = (kind == 0) ? nemo_libm::logf_glibc(in[i])
+			: (kind == 1) ? nemo_libm::sinf_glibc(in[i]) : nemo_libm::kuramoto_wrap(in[i]);
 	}
 }
 

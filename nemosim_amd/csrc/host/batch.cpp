@@ -116,7 +116,7 @@ void
 runBatch(Simulation* const sims[], size_t n, const unsigned nsteps[], unsigned long long events[])
 {
 	std::lock_guard<std::mutex> lock(g_planMutex);
-	std::lock_guard<std::recursive_mutex> devLock(deviceMutex());
+	std::lock_guard<DeviceMutex> devLock(deviceMutex());
 	/* Plugin-path simulations and empty work run on their own; the rest are
 	 * grouped by (device, kernel variant) and, within a group, by workgroup
 	 * size, keeping the given order within each size, then packed greedily

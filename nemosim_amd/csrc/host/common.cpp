@@ -83,6 +83,12 @@ pluginPaths()
 	return p;
 }
 
+static bool
+isTrue(const std::string& v)
+{
+	return v == "true" || v == "1" || v == "yes" || v == "on";
+}
+
 /* Minimal reader of NeMo's model descriptor format: key=value lines,
  * [section] headers, '#' comments (src/nemo/NeuronType.cpp:92-155). */
 static bool
@@ -115,7 +121,15 @@ readDescriptor(const std::string& path, ModelInfo& m)
 		if(section.empty() && key == "parameters") { m.nparam = unsigned(std::atoi(val.c_str())); haveP = true; }
 		else if(section.empty() && key == "state-variables") { m.nstate = unsigned(std::atoi(val.c_str())); haveS = true; }
 		else if(section.empty() && key == "membrane-potential") m.membranePotential = unsigned(std::atoi(val.c_str()));
-		else if(section == "rng" && key == "normal") m.normalRng = (val == "true" || val == "1");
+		else if(section.empty() && key == "history") m.history = unsigned(std::atoi(val.c_str()));
+		else if(section == "rng" && key == "normal") m.normalRng = isTrue(val);
+		else if(section == "rcm" && key == "sources") m.rcmSources = isTrue(val);
+		else if(section == "rcm" && key == "delays") m.rcmDelays = isTrue(val);
+		else if(section == "rcm" && key == "forward") m.rcmForward = isTrue(val);
+		else if(section == "rcm" && key == "weights") m.rcmWeights = isTrue(val);
+	}
+	if(m.history < 1) {
+		throw exception(NEMO_DL_ERROR, fmt("Model descriptor ", path, " has history=0; at least one cycle is kept"));
 	}
 	if(!haveP || !haveS) {
 		throw exception(NEMO_DL_ERROR, fmt("Model descriptor ", path, " lacks parameters= or state-variables="));
@@ -186,11 +200,37 @@ hipCheck(int err, const char* what)
 	}
 }
 
-std::recursive_mutex&
+void
+DeviceMutex::lock()
+{
+	std::unique_lock<std::mutex> g(m_);
+	const std::thread::id me = std::this_thread::get_id();
+	if(m_depth > 0 && m_owner == me) {
+		m_depth += 1;
+		return;
+	}
+	const uint64_t ticket = m_next++;
+	cv_.wait(g, [&] { return m_depth == 0 && m_serving == ticket; });
+	m_owner = me;
+	m_depth = 1;
+}
+
+void
+DeviceMutex::unlock()
+{
+	std::lock_guard<std::mutex> g(m_);
+	if(--m_depth == 0) {
+		m_owner = std::thread::id();
+		m_serving += 1;
+		cv_.notify_all();
+	}
+}
+
+DeviceMutex&
 deviceMutex()
 {
 	/* never destroyed: threads may still hold it while the process exits */
-	static std::recursive_mutex* m = new std::recursive_mutex();
+	static DeviceMutex* m = new DeviceMutex();
 	return *m;
 }
 

@@ -1,7 +1,9 @@
 #ifndef NEMOSIM_HOST_COMMON_HPP
 #define NEMOSIM_HOST_COMMON_HPP
 
+#include <condition_variable>
 #include <mutex>
+#include <thread>
 #include <stdexcept>
 #include <string>
 #include <sstream>
@@ -33,11 +35,11 @@ std::string fmt(Args&&... args)
 }
 
 /* A neuron model: its descriptor (NeMo's <Model>.ini: parameters=,
- * state-variables=, membrane-potential=, [rng] normal=,
- * src/nemo/NeuronType.cpp:92-155) and where its plugin lives. Built-in
- * models (Izhikevich, IF_curr_exp, IF_curr_exp_noise) also run in the fused
- * step kernel; any other model found on the plugin path runs through its
- * plugin's hip_update_neurons. */
+ * state-variables=, membrane-potential=, history=, [rcm] sources= delays=
+ * forward= weights=, [rng] normal=, src/nemo/NeuronType.cpp:92-155) and
+ * where its plugin lives. Built-in models (Izhikevich, IF_curr_exp,
+ * IF_curr_exp_noise) also run in the fused step kernel; any other model
+ * found on the plugin path runs through its plugin's hip_update_neurons. */
 struct ModelInfo {
 	std::string name;
 	unsigned id;
@@ -47,6 +49,12 @@ struct ModelInfo {
 	bool normalRng;
 	bool builtin;
 	std::string dir;      /* directory of the descriptor; plugin in dir/hip/ */
+	unsigned history = 1;          /* cycles of state kept */
+	bool rcmSources = false;       /* reverse-matrix fields the model reads */
+	bool rcmDelays = false;
+	bool rcmForward = false;
+	bool rcmWeights = false;
+	bool usesRcm() const { return rcmSources || rcmDelays || rcmForward || rcmWeights; }
 };
 
 /* Resolve a model name: built-ins first, then <dir>/<name>.ini on the plugin
@@ -67,13 +75,34 @@ float fx_toFloat(int32_t v);
 
 void hipCheck(int err, const char* what);
 
+/* A recursive mutex that is granted in the order it was asked for: a
+ * thread that releases it and asks again queues behind the threads already
+ * waiting, so a loop of launches on one thread cannot starve uploads queued
+ * by others. */
+class DeviceMutex
+{
+	public:
+		void lock();
+		void unlock();
+
+	private:
+		std::mutex m_;
+		std::condition_variable cv_;
+		uint64_t m_next = 0;        /* next ticket to hand out */
+		uint64_t m_serving = 0;     /* ticket whose holder may take it */
+		unsigned m_depth = 0;
+		std::thread::id m_owner;
+};
+
 /* Held around this library's device work -- uploads at construction,
  * launches, copies -- so that work queued from one host thread (a serving
  * loop building simulations on worker threads) never shares the device
  * with a persistent step launch: its workgroups must all be resident at
  * once, and a copy kernel dispatched into the middle of that launch could
- * delay some of them. Recursive: a batched run launches single runs. */
-std::recursive_mutex& deviceMutex();
+ * delay some of them. Recursive: a batched run launches single runs. First
+ * come, first served: a serving loop that launches chunk after chunk lets
+ * the uploads of the networks built meanwhile in between. */
+DeviceMutex& deviceMutex();
 
 } // namespace nemo
 

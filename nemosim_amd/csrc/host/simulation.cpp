@@ -213,7 +213,7 @@ Simulation::Simulation(const NetworkImpl& net, const ConfigurationImpl& conf) :
 	const std::vector<unsigned char>& Pl = net.plastic();
 	m_nsyn = S.size();
 	hipStream_t bs = static_cast<hipStream_t>(m_stream);
-	std::unique_lock<std::recursive_mutex> buildLock(deviceMutex(), std::defer_lock);
+	std::unique_lock<DeviceMutex> buildLock(deviceMutex(), std::defer_lock);
 	DevBuf dS, dTg, dW, dPl, dDl, dLsrc, dLtgt, dWfx;
 	bool onDevice = conf.deviceConstruction && m_nsyn > 0 && m_nsyn < (size_t(1) << 32) && m_n > 0;
 	uint32_t bflags = 0;
@@ -293,6 +293,44 @@ Simulation::Simulation(const NetworkImpl& net, const ConfigurationImpl& conf) :
 		if(!g.gidx.empty() && !g.model->builtin) allBuiltin = false;
 		m_np = std::max(m_np, g.model->nparam);
 		m_ns = std::max(m_ns, g.model->nstate);
+	}
+	/* models that keep state history or read the reverse matrix (Kuramoto).
+	 * src/nemo/construction/RCM.cpp:90-97 refuses STDP with such a type.
+	 * The reference's CPU Kuramoto plugin indexes its arrays with
+	 * simulation-wide indices (src/nemo/cpu/plugins/Kuramoto.cpp:105-110) and
+	 * its CUDA plugin asserts that the type fills the whole network
+	 * (src/nemo/cuda/plugins/Kuramoto.cu:270): a type reading the reverse
+	 * matrix must be the network's only type here. A delay d reads the
+	 * plane of cycle t-(d-1), which a history of H cycles holds while
+	 * d <= H-1 (src/nemo/cuda/plugins/Kuramoto.cu:116). */
+	{
+		bool rcmType = false, otherType = false;
+		unsigned rcmHistory = 0;
+		for(const NeuronGroup& g : net.groups) {
+			if(g.gidx.empty()) continue;
+			m_hist = std::max(m_hist, g.model->history);
+			if(g.model->usesRcm()) {
+				rcmType = true;
+				if(g.model->rcmDelays) rcmHistory = std::max(rcmHistory, g.model->history);
+			} else {
+				otherType = true;
+			}
+		}
+		if(m_plane.size() <= 255) m_plane.assign(256, 0u);
+		if(rcmType && m_stdp) {
+			throw exception(NEMO_API_UNSUPPORTED,
+					"The current version does not support a mix of STDP with neuron types which require reverse connectivity for other purposes");
+		}
+		if(rcmType && otherType) {
+			throw exception(NEMO_API_UNSUPPORTED,
+					"a neuron type that reads the reverse connectivity (e.g. Kuramoto) must be the only type in the network");
+		}
+		if(rcmHistory > 1 && net.maxDelay() >= rcmHistory) {
+			throw exception(NEMO_INVALID_INPUT, fmt("delays of up to ", rcmHistory - 1,
+						" cycles are supported for a model with a state history of ", rcmHistory,
+						" cycles; the network has a delay of ", net.maxDelay()));
+		}
+		m_hasRcm = rcmType;
 	}
 	std::string whyNotFused;
 	if(conf.stepMode == NEMO_STEP_PLUGINS) {
@@ -505,6 +543,48 @@ Simulation::Simulation(const NetworkImpl& net, const ConfigurationImpl& conf) :
 			}
 		}
 
+		/* ---- reverse matrix of every synapse for a model that reads it
+		 * (src/nemo/construction/RCM.cpp:150-171 with STDP off): per target,
+		 * in the order of the network's synapse iterator -- sources ascending
+		 * by neuron index, each source's synapses in insertion order
+		 * (src/nemo/NetworkImpl.cpp:311-325, src/nemo/ConnectivityMatrix.cpp:79-87) */
+		if(m_hasRcm) {
+			std::vector<size_t> order(m_nsyn);
+			for(size_t i = 0; i < m_nsyn; ++i) order[i] = i;
+			std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return S[a] < S[b]; });
+			std::vector<uint32_t> kStart(size_t(m_n) + 1, 0u), kSrc(m_nsyn), kDelay(m_nsyn), kFwd(m_nsyn);
+			std::vector<float> kW(m_nsyn);
+			for(size_t i = 0; i < m_nsyn; ++i) kStart[ltgt[i] + 1] += 1;
+			for(uint32_t l = 0; l < m_n; ++l) kStart[l + 1] += kStart[l];
+			std::vector<uint32_t> kc(kStart.begin(), kStart.end() - 1);
+			for(size_t k = 0; k < m_nsyn; ++k) {
+				const size_t i = order[k];
+				const uint32_t r = kc[ltgt[i]]++;
+				kSrc[r] = lsrc[i];
+				kDelay[r] = Dl[i];
+				kW[r] = W[i];
+				kFwd[r] = fwd[i];
+			}
+			bool wantSrc = false, wantDelay = false, wantW = false, wantFwd = false;
+			for(const NeuronGroup& g : net.groups) {
+				if(g.gidx.empty()) continue;
+				wantSrc = wantSrc || g.model->rcmSources;
+				wantDelay = wantDelay || g.model->rcmDelays;
+				wantW = wantW || g.model->rcmWeights;
+				wantFwd = wantFwd || g.model->rcmForward;
+			}
+			std::lock_guard<DeviceMutex> uploadLock(deviceMutex());
+			std::memset(&m_rcmDesc, 0, sizeof(m_rcmDesc));
+			m_rcmDesc.targets = m_n;
+			m_rcmDesc.synapses = m_nsyn;
+			d_kStart.upload(kStart.data(), kStart.size() * 4);
+			m_rcmDesc.start = d_kStart.as<uint32_t>();
+			if(wantSrc && m_nsyn) { d_kSrc.upload(kSrc.data(), m_nsyn * 4); m_rcmDesc.source = d_kSrc.as<uint32_t>(); }
+			if(wantDelay && m_nsyn) { d_kDelay.upload(kDelay.data(), m_nsyn * 4); m_rcmDesc.delay = d_kDelay.as<uint32_t>(); }
+			if(wantW && m_nsyn) { d_kW.upload(kW.data(), m_nsyn * 4); m_rcmDesc.weight = d_kW.as<float>(); }
+			if(wantFwd && m_nsyn) { d_kFwd.upload(kFwd.data(), m_nsyn * 4); m_rcmDesc.forward = d_kFwd.as<uint32_t>(); }
+		}
+
 		/* ---- reverse matrix of plastic synapses (src/nemo/construction/RCM.cpp:150-171) */
 		if(m_stdp) {
 			rcmStart.assign(size_t(m_n) + 1, 0);
@@ -528,7 +608,9 @@ Simulation::Simulation(const NetworkImpl& net, const ConfigurationImpl& conf) :
 
 	/* ---- neuron arrays */
 	std::vector<float> param(size_t(m_np) * m_n, 0.0f);
-	std::vector<float> state(size_t(m_ns) * m_n, 0.0f);
+	/* plane 0 holds the initial state; the others, for models with history,
+	 * are filled by the plugin's init */
+	std::vector<float> state(size_t(m_hist) * m_ns * m_n, 0.0f);
 	{
 		uint32_t l = 0;
 		for(const NeuronGroup& g : net.groups) {
@@ -549,7 +631,7 @@ Simulation::Simulation(const NetworkImpl& net, const ConfigurationImpl& conf) :
 	}
 
 	/* ---- upload */
-	std::unique_lock<std::recursive_mutex> devLock(deviceMutex());
+	std::unique_lock<DeviceMutex> devLock(deviceMutex());
 	d_model.upload(m_model.data(), m_model.size());
 	d_param.upload(param.data(), param.size() * sizeof(float));
 	d_state.upload(state.data(), state.size() * sizeof(float));
@@ -637,7 +719,7 @@ Simulation::buildForwardOnDevice(const std::vector<unsigned>& Tg, const std::vec
 		const std::vector<unsigned char>& Pl, uint32_t bflags, const uint32_t* lsrc, const uint32_t* ltgt,
 		const int32_t* wfx, const uint8_t* dDl, const uint8_t* dPl)
 {
-	std::lock_guard<std::recursive_mutex> lock(deviceMutex());
+	std::lock_guard<DeviceMutex> lock(deviceMutex());
 	hipStream_t st = static_cast<hipStream_t>(m_stream);
 	const uint32_t P1 = m_P + 1;
 	const size_t nb = size_t(m_n) * P1;
@@ -784,6 +866,8 @@ Simulation::loadPlugins(const NetworkImpl& net)
 		PluginGroup pg;
 		pg.start = base;
 		pg.end = base + cnt;
+		pg.model = g.model->id;
+		pg.history = g.model->history;
 		pg.handle = h;
 		pg.update = reinterpret_cast<hip_update_neurons_t*>(dlsym(h, "hip_update_neurons"));
 		hip_init_neurons_t* init = reinterpret_cast<hip_init_neurons_t*>(dlsym(h, "hip_init_neurons"));
@@ -793,7 +877,7 @@ Simulation::loadPlugins(const NetworkImpl& net)
 		}
 		m_plugins.push_back(pg);
 		hipCheck(init(m_stream, pg.start, pg.end, d_param.as<float>() + pg.start, m_n,
-					d_state.as<float>() + pg.start, m_n, d_rng.as<uint32_t>() + pg.start, m_n),
+					d_state.as<float>() + pg.start, size_t(m_ns) * m_n, m_n, d_rng.as<uint32_t>() + pg.start, m_n),
 				"hip_init_neurons");
 		base += cnt;
 	}
@@ -801,7 +885,7 @@ Simulation::loadPlugins(const NetworkImpl& net)
 
 Simulation::~Simulation()
 {
-	std::lock_guard<std::recursive_mutex> devLock(deviceMutex());
+	std::lock_guard<DeviceMutex> devLock(deviceMutex());
 	if(m_stream) {
 		(void) hipStreamSynchronize(static_cast<hipStream_t>(m_stream));
 		(void) hipStreamDestroy(static_cast<hipStream_t>(m_stream));
@@ -875,7 +959,7 @@ Simulation::translateSchedule(unsigned nsteps, const nemo_schedule_t* s, std::ve
 void
 Simulation::uploadSchedule(unsigned nsteps, const nemo_schedule_t* s)
 {
-	std::lock_guard<std::recursive_mutex> devLock(deviceMutex());
+	std::lock_guard<DeviceMutex> devLock(deviceMutex());
 	m_hasSchedule = false;
 	std::vector<uint32_t> fsPtr, fsIdx, isPtr, isIdx;
 	std::vector<float> isVal;
@@ -1078,7 +1162,7 @@ Simulation::completeLaunch(unsigned nsteps, const uint32_t ctl[8])
 void
 Simulation::launchPlugins(unsigned nsteps, unsigned long long* events)
 {
-	std::lock_guard<std::recursive_mutex> devLock(deviceMutex());
+	std::lock_guard<DeviceMutex> devLock(deviceMutex());
 	hipStream_t st = static_cast<hipStream_t>(m_stream);
 	hipCheck(hipSetDevice(m_device), "hipSetDevice");
 	ensureRaster(size_t(nsteps) * m_words32 * 4);
@@ -1128,11 +1212,13 @@ Simulation::launchPlugins(unsigned nsteps, unsigned long long* events)
 			hipCheck(nemo_dev_pm_stimulus(&s, k, entries, st), "plugin step: stimulus");
 		}
 		uint32_t* firedRow = s.raster + size_t(k) * m_words32;
+		/* the cycle a plugin sees is the timer's (src/nemo/cpu/Simulation.cpp:98) */
+		const unsigned tcycle = unsigned(m_timerCycles + k);
 		for(const PluginGroup& pg : m_plugins) {
-			hipCheck(pg.update(m_stream, unsigned(m_cycle + k), pg.start, pg.end,
-						d_param.as<float>() + pg.start, m_n, d_state.as<float>() + pg.start, m_n,
+			hipCheck(pg.update(m_stream, tcycle, pg.start, pg.end,
+						d_param.as<float>() + pg.start, m_n, d_state.as<float>() + pg.start, size_t(m_ns) * m_n, m_n,
 						d_rng.as<uint32_t>() + pg.start, m_n, s.fstim, s.curE, s.curI, s.curX,
-						s.recent, firedRow), "hip_update_neurons");
+						s.recent, firedRow, m_hasRcm ? &m_rcmDesc : nullptr), "hip_update_neurons");
 		}
 		hipCheck(nemo_dev_pm_compact(&s, k, st), "plugin step: compact");
 		hipCheck(nemo_dev_pm_scatter(&s, k, st), "plugin step: scatter");
@@ -1149,6 +1235,13 @@ Simulation::launchPlugins(unsigned nsteps, unsigned long long* events)
 	float ms = 0.0f;
 	hipCheck(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
 	m_kernelMs = ms;
+	/* the last update of the run, of timer cycle c, made plane (c+1) mod H
+	 * current (src/nemo/cpu/Neurons.cpp:123) */
+	for(const PluginGroup& pg : m_plugins) {
+		if(pg.history > 1) {
+			m_plane.at(pg.model) = (unsigned(m_timerCycles + nsteps - 1) + 1u) % pg.history;
+		}
+	}
 	m_cycle += nsteps;
 	m_timerCycles += nsteps;
 	m_lastRunSteps = nsteps;
@@ -1163,7 +1256,7 @@ Simulation::launchPlugins(unsigned nsteps, unsigned long long* events)
 void
 Simulation::launch(unsigned nsteps, unsigned long long* events)
 {
-	std::lock_guard<std::recursive_mutex> devLock(deviceMutex());
+	std::lock_guard<DeviceMutex> devLock(deviceMutex());
 	if(events) *events = 0;
 	if(!m_fused && m_n > 0 && nsteps > 0) {
 		launchPlugins(nsteps, events);
@@ -1220,7 +1313,7 @@ void
 Simulation::runStatic(unsigned nsteps, const nemo_schedule_t* schedule, uint64_t* rasterDev,
 		unsigned long long* events)
 {
-	std::lock_guard<std::recursive_mutex> devLock(deviceMutex());
+	std::lock_guard<DeviceMutex> devLock(deviceMutex());
 	if(events) *events = 0;
 	if(!m_fused) {
 		throw exception(NEMO_API_UNSUPPORTED,
@@ -1293,7 +1386,7 @@ Simulation::decodeStep(const uint64_t* words, std::vector<unsigned>& out) const
 const std::vector<unsigned>&
 Simulation::step(const std::vector<unsigned>& fstim, const unsigned* istimIdx, const float* istimVal, size_t nistim)
 {
-	std::lock_guard<std::recursive_mutex> devLock(deviceMutex());
+	std::lock_guard<DeviceMutex> devLock(deviceMutex());
 	unsigned fp[2] = { 0, unsigned(fstim.size()) };
 	unsigned ip[2] = { 0, unsigned(nistim) };
 	nemo_schedule_t s;
@@ -1331,7 +1424,7 @@ Simulation::run(unsigned nsteps, const nemo_schedule_t* schedule, unsigned long 
 void
 Simulation::readFired(unsigned counts[], unsigned ids[], size_t cap, size_t* total)
 {
-	std::lock_guard<std::recursive_mutex> devLock(deviceMutex());
+	std::lock_guard<DeviceMutex> devLock(deviceMutex());
 	requireOwnRaster();
 	size_t nw = size_t(m_lastRunSteps) * m_words;
 	m_rasterHost.resize(nw);
@@ -1357,7 +1450,7 @@ Simulation::readFired(unsigned counts[], unsigned ids[], size_t cap, size_t* tot
 void
 Simulation::readRaster(uint64_t words[], size_t cap, size_t* wordsPerStep)
 {
-	std::lock_guard<std::recursive_mutex> devLock(deviceMutex());
+	std::lock_guard<DeviceMutex> devLock(deviceMutex());
 	size_t nw = size_t(m_lastRunSteps) * m_words;
 	if(wordsPerStep) *wordsPerStep = m_words;
 	if(words) {
@@ -1405,7 +1498,7 @@ Simulation::info(nemo_hip_info_t* out) const
 void
 Simulation::applyStdp(float reward)
 {
-	std::lock_guard<std::recursive_mutex> devLock(deviceMutex());
+	std::lock_guard<DeviceMutex> devLock(deviceMutex());
 	if(!m_stdp) {
 		throw exception(NEMO_LOGIC_ERROR, "applyStdp called, but no STDP model specified");
 	}
@@ -1426,7 +1519,7 @@ Simulation::applyStdp(float reward)
 void
 Simulation::readScalar(const DevBuf& b, size_t offset, void* out, size_t bytes) const
 {
-	std::lock_guard<std::recursive_mutex> devLock(deviceMutex());
+	std::lock_guard<DeviceMutex> devLock(deviceMutex());
 	hipCheck(hipSetDevice(m_device), "hipSetDevice");
 	hipCheck(hipMemcpy(out, static_cast<const char*>(b.ptr) + offset, bytes, hipMemcpyDeviceToHost), "D2H");
 }
@@ -1434,7 +1527,7 @@ Simulation::readScalar(const DevBuf& b, size_t offset, void* out, size_t bytes) 
 void
 Simulation::writeScalar(DevBuf& b, size_t offset, const void* in, size_t bytes)
 {
-	std::lock_guard<std::recursive_mutex> devLock(deviceMutex());
+	std::lock_guard<DeviceMutex> devLock(deviceMutex());
 	hipCheck(hipSetDevice(m_device), "hipSetDevice");
 	hipCheck(hipMemcpy(static_cast<char*>(b.ptr) + offset, in, bytes, hipMemcpyHostToDevice), "H2D");
 }
@@ -1447,8 +1540,17 @@ Simulation::getNeuronState(unsigned g, unsigned var) const
 		throw exception(NEMO_INVALID_INPUT, fmt("Invalid state variable index ", var));
 	}
 	float v;
-	readScalar(d_state, (size_t(var) * m_n + l) * 4, &v, 4);
+	readScalar(d_state, stateOffset(l, var) * 4, &v, 4);
 	return v;
+}
+
+/* element of state variable var of neuron l in its type's current plane */
+size_t
+Simulation::stateOffset(unsigned l, unsigned var) const
+{
+	const ModelInfo& m = modelInfo(m_model[l]);
+	const size_t plane = m.history > 1 ? m_plane.at(m.id) : 0u;
+	return (plane * m_ns + var) * m_n + l;
 }
 
 float
@@ -1480,6 +1582,8 @@ Simulation::setNeuron(unsigned g, unsigned nargs, const float args[])
 					m.nparam + m.nstate, ", found ", nargs));
 	}
 	for(unsigned k = 0; k < m.nparam; ++k) writeScalar(d_param, (size_t(k) * m_n + l) * 4, &args[k], 4);
+	/* src/nemo/cpu/Neurons.cpp:66-81: the state goes to plane 0, whichever
+	 * plane is current */
 	for(unsigned k = 0; k < m.nstate; ++k) writeScalar(d_state, (size_t(k) * m_n + l) * 4, &args[m.nparam + k], 4);
 }
 
@@ -1490,7 +1594,7 @@ Simulation::setNeuronState(unsigned g, unsigned var, float val)
 	if(var >= modelInfo(m_model[l]).nstate) {
 		throw exception(NEMO_INVALID_INPUT, fmt("Invalid state variable index ", var));
 	}
-	writeScalar(d_state, (size_t(var) * m_n + l) * 4, &val, 4);
+	writeScalar(d_state, stateOffset(l, var) * 4, &val, 4);
 }
 
 void

@@ -131,10 +131,20 @@ class Simulation
 		/* plugin path */
 		struct PluginGroup {
 			unsigned start, end;
+			unsigned model, history;
 			void* handle;
 			hip_update_neurons_t* update;
 		};
 		std::vector<PluginGroup> m_plugins;
+		/* state planes: [m_hist][m_ns][m_n]; the current plane of each model
+		 * with history > 1 (by model id; src/nemo/cpu/Neurons.cpp:123) */
+		uint32_t m_hist = 1;
+		std::vector<uint32_t> m_plane;
+		size_t stateOffset(unsigned l, unsigned var) const;
+		/* reverse matrix of every synapse for models that read it */
+		DevBuf d_kStart, d_kSrc, d_kDelay, d_kW, d_kFwd;
+		nemo_hip_rcm m_rcmDesc = {};
+		bool m_hasRcm = false;
 		uint32_t m_words32 = 0;
 		DevBuf d_acc, d_curE, d_curI, d_curX, d_fstim, d_list;
 		std::vector<uint32_t> m_fsPtrHost, m_isPtrHost;

@@ -51,12 +51,13 @@ __global__ void plugin_update_kernel(unsigned start, unsigned end,
 
 extern "C" __attribute__((visibility("default")))
 int hip_update_neurons(void* stream, unsigned cycle, unsigned start, unsigned end,
-		float* param, size_t paramStride, float* state, size_t stateStride,
+		float* param, size_t paramStride, float* state, size_t stateHistoryStride, size_t stateStride,
 		uint32_t* rng, size_t rngStride, const uint32_t* fstim,
 		const float* currentE, const float* currentI, const float* currentExt,
-		uint64_t* recentFiring, uint32_t* fired)
+		uint64_t* recentFiring, uint32_t* fired, const nemo_hip_rcm* rcm)
 {
-	(void) cycle;   /* history length 1: the cycle selects no state plane */
+	/* history length 1: the cycle selects no state plane; no reverse matrix */
+	(void) cycle; (void) stateHistoryStride; (void) rcm;
 	if(end <= start) return 0;
 	const unsigned n = end - start;
 	hipLaunchKernelGGL(plugin_update_kernel, dim3((n + 255) / 256), dim3(256), 0,
@@ -69,11 +70,11 @@ int hip_update_neurons(void* stream, unsigned cycle, unsigned start, unsigned en
  * with a history of one cycle */
 extern "C" __attribute__((visibility("default")))
 int hip_init_neurons(void* stream, unsigned start, unsigned end,
-		float* param, size_t paramStride, float* state, size_t stateStride,
+		float* param, size_t paramStride, float* state, size_t stateHistoryStride, size_t stateStride,
 		uint32_t* rng, size_t rngStride)
 {
 	(void) stream; (void) start; (void) end; (void) param; (void) paramStride;
-	(void) state; (void) stateStride; (void) rng; (void) rngStride;
+	(void) state; (void) stateHistoryStride; (void) stateStride; (void) rng; (void) rngStride;
 	return 0;
 }
 

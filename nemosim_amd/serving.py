@@ -45,6 +45,7 @@ class Request(object):
         self.reward = float(reward)
         # filled by the server
         self.admitted_s = None
+        self.built_s = None
         self.done_s = None
         self.cycles_done = 0
         self.events = 0
@@ -145,6 +146,7 @@ class Server(object):
         self.used = 0
         self.lock = threading.Lock()
         self.gpu_busy_s = 0.0
+        self.depth = []       # (time, waiting, building, resident) at each pass of the loop
 
     def _transient(self, r):
         """Device memory a request holds only while it is built."""
@@ -211,11 +213,13 @@ class Server(object):
             for fut, r in building:
                 if fut.done():
                     r.sim = fut.result()
+                    r.built_s = now()
                     self.used -= self._transient(r)
                     active.append(r)
                 else:
                     still.append((fut, r))
             building = still
+            self.depth.append((now(), len(waiting), len(building), len(active)))
             if active:
                 self._advance(active)
                 finished = [r for r in active if r.cycles_done >= r.duration_ms]

@@ -170,25 +170,29 @@ initialiseRng(unsigned minIdx, unsigned maxIdx, std::vector<RNG>& rngs)
 /* ---------------------------------------------------------- neuron types */
 
 enum ModelId { MODEL_IZHIKEVICH = 0, MODEL_IF_CURR_EXP = 1, MODEL_IF_CURR_EXP_NOISE = 2,
-	MODEL_POISSON_SOURCE = 3, MODEL_INPUT = 4 };
-static const unsigned NMODELS = 5;
+	MODEL_POISSON_SOURCE = 3, MODEL_INPUT = 4, MODEL_KURAMOTO = 5 };
+static const unsigned NMODELS = 6;
 
 struct TypeInfo {
 	const char* name;
 	unsigned nparam;
 	unsigned nstate;
 	unsigned membranePotential;
+	unsigned history;      /* cycles of state kept (planes of the state array) */
+	bool rcm;              /* reads the reverse connectivity (sources, delays, weights) */
 };
 
 /* src/nemo/plugins/Izhikevich.ini:1-5, src/nemo/plugins/IF_curr_exp.ini:14-26,
- * src/nemo/plugins/PoissonSource.ini:1-4, src/nemo/plugins/Input.ini:1-4;
- * IF_curr_exp_noise is IF_curr_exp plus a 10th parameter sigma. */
+ * src/nemo/plugins/PoissonSource.ini:1-4, src/nemo/plugins/Input.ini:1-4,
+ * src/nemo/plugins/Kuramoto.ini:2-12; IF_curr_exp_noise is IF_curr_exp plus
+ * a 10th parameter sigma. */
 static const TypeInfo TYPES[NMODELS] = {
-	{ "Izhikevich", 5, 2, 1 },
-	{ "IF_curr_exp", 9, 4, 0 },
-	{ "IF_curr_exp_noise", 10, 4, 0 },
-	{ "PoissonSource", 1, 0, 0 },
-	{ "Input", 0, 0, 0 },
+	{ "Izhikevich", 5, 2, 1, 1, false },
+	{ "IF_curr_exp", 9, 4, 0, 1, false },
+	{ "IF_curr_exp_noise", 10, 4, 0, 1, false },
+	{ "PoissonSource", 1, 0, 0, 1, false },
+	{ "Input", 0, 0, 0, 1, false },
+	{ "Kuramoto", 1, 1, 0, 32, true },
 };
 
 /* Parameter / state indices: src/nemo/plugins/Izhikevich.h:6-13 */
@@ -440,6 +444,89 @@ struct StdpProcess {
 	}
 };
 
+/* Kuramoto oscillators, src/nemo/cpu/plugins/Kuramoto.cpp. Parameter 0 is
+ * the natural frequency (radians per cycle), state 0 the phase, kept for
+ * KURAMOTO_HISTORY cycles: plane c mod KURAMOTO_HISTORY holds the phase of
+ * cycle c (src/nemo/plugins/Kuramoto.h:4). The incoming couplings of each
+ * oscillator are its reverse-matrix entries in the reference's order
+ * (kStart/kSource/kDelay/kWeight, sim-local indices, built in finalize). */
+static const unsigned KURAMOTO_HISTORY = 32;
+
+/* Kuramoto.cpp:37-42 */
+static inline float*
+kuramoto_phase(float* base, size_t stride, unsigned cycle)
+{
+	return base + size_t(cycle % KURAMOTO_HISTORY) * stride;
+}
+
+/* Kuramoto.cpp:121 and :150 -- the remainder is float, the constant
+ * 2.0f*M_PI of the conditional is a double, so for a negative phase the sum
+ * is formed in double and rounded to float on assignment */
+static inline float
+kuramoto_wrap(float phase)
+{
+	return fmodf(phase, 2.0f * M_PI) + (phase < 0.0f ? 2.0f * M_PI : 0.0f);
+}
+
+/* Kuramoto.cpp:18-29: sum of w * sin(source - target) in reverse-matrix order */
+static inline float
+kuramoto_sum(const std::vector<float>& weight, const std::vector<float>& sourcePhase, float targetPhase)
+{
+	float sum = 0.0f;
+	for(size_t i = 0; i < weight.size(); ++i) {
+		sum += weight[i] * sinf(sourcePhase[i] - targetPhase);
+	}
+	return sum;
+}
+
+/* Kuramoto.cpp:85-123: fourth-order Runge-Kutta step of one cycle; the
+ * source phases are those of cycle `cycle - (delay - 1)` */
+static void
+update_kuramoto(unsigned start, unsigned end, unsigned cycle, const float* frequency,
+		float* stateBase, size_t historyStride, const std::vector<size_t>& kStart,
+		const std::vector<unsigned>& kSource, const std::vector<unsigned>& kDelay,
+		const std::vector<float>& kWeight)
+{
+	const float* phase0 = kuramoto_phase(stateBase, historyStride, cycle);
+	float* phase1 = kuramoto_phase(stateBase, historyStride, cycle + 1u);
+	/* the oscillators are independent within a cycle (delays <= 31 never
+	 * read the plane being written), so they may be split over threads */
+#pragma omp parallel for schedule(static)
+	for(unsigned n = start; n < end; ++n) {
+		std::vector<float> weight, sourcePhase;
+		const float f = frequency[n];
+		float targetPhase = phase0[n];
+		/* Kuramoto.cpp:48-80 (loadIncoming) */
+		weight.clear();
+		sourcePhase.clear();
+		for(size_t r = kStart[n]; r < kStart[n + 1]; ++r) {
+			weight.push_back(kWeight[r]);
+			sourcePhase.push_back(kuramoto_phase(stateBase, historyStride, cycle - int(kDelay[r] - 1))[kSource[r]]);
+		}
+		float k0 = f + kuramoto_sum(weight, sourcePhase, targetPhase);
+		float k1 = f + kuramoto_sum(weight, sourcePhase, targetPhase + k0 * 0.5f);
+		float k2 = f + kuramoto_sum(weight, sourcePhase, targetPhase + k1 * 0.5f);
+		float k3 = f + kuramoto_sum(weight, sourcePhase, targetPhase + k2);
+		targetPhase += (k0 + 2 * k1 + 2 * k2 + k3) / 6.0f;
+		phase1[n] = kuramoto_wrap(targetPhase);
+	}
+}
+
+/* Kuramoto.cpp:133-152: the history before cycle 0, by running the
+ * uncoupled model backwards from the initial phase in plane 0 */
+static void
+init_kuramoto(unsigned start, unsigned end, const float* frequency, float* stateBase, size_t historyStride)
+{
+	for(unsigned t = 0; t < KURAMOTO_HISTORY - 1u; ++t) {
+		const float* phase0 = kuramoto_phase(stateBase, historyStride, 0u - t);
+		float* phase1 = kuramoto_phase(stateBase, historyStride, 0u - (t + 1u));
+		for(unsigned n = start; n < end; ++n) {
+			const float phase = phase0[n] - frequency[n];
+			phase1[n] = kuramoto_wrap(phase);
+		}
+	}
+}
+
 /* -------------------------------------------------------------- network */
 
 struct Synapse {
@@ -484,7 +571,15 @@ struct Oracle {
 	std::unordered_map<unsigned, unsigned> globalToLocal;
 	std::vector<unsigned> localGroup;
 	std::vector<std::vector<float> > simParam;       // per group: [nparam*n]
-	std::vector<std::vector<float> > simState;       // per group: [nstate*n]
+	std::vector<std::vector<float> > simState;       // per group: [history][nstate][n]
+	std::vector<unsigned> stateCurrent;              // per group: plane of the current state
+
+	/* reverse matrix of every synapse, for the Kuramoto model
+	 * (src/nemo/construction/RCM.cpp:150-171 with STDP off; per target in
+	 * the order of the network's synapse iterator) */
+	std::vector<size_t> kStart;
+	std::vector<unsigned> kSource, kDelay;
+	std::vector<float> kWeight;
 	std::vector<std::vector<RNG> > simRng;
 
 	std::vector<unsigned> fired;
@@ -609,11 +704,38 @@ struct Oracle {
 				throw exception(NEMO_INVALID_INPUT, "Max delay + postfire part of STDP function must be <= 64");
 			}
 		}
+		/* Kuramoto oscillators. src/nemo/construction/RCM.cpp:90-97 refuses
+		 * STDP together with a type that reads the reverse matrix. The two
+		 * further conditions are this restatement's: the reference's CPU
+		 * plugin indexes its arrays with simulation-wide indices
+		 * (Kuramoto.cpp:105-110), which stays inside them only when the
+		 * oscillators are the network's only neurons (the CUDA plugin asserts
+		 * the same, src/nemo/cuda/plugins/Kuramoto.cu:270), and a delay of
+		 * KURAMOTO_HISTORY or more would read the plane being written this
+		 * cycle (Kuramoto.cu:116 asserts delay - 1 < KURAMOTO_HISTORY - 1). */
+		bool rcmType = false, otherType = false;
+		for(size_t gi = 0; gi < groups.size(); ++gi) {
+			if(groups[gi].gidx.empty()) continue;
+			if(TYPES[groups[gi].model].rcm) rcmType = true; else otherType = true;
+		}
+		if(rcmType && hasStdp) {
+			throw exception(NEMO_API_UNSUPPORTED,
+					"The current version does not support a mix of STDP with neuron types which require reverse connectivity for other purposes");
+		}
+		if(rcmType && otherType) {
+			throw exception(NEMO_API_UNSUPPORTED, "Kuramoto oscillators cannot share a network with other neuron types");
+		}
+		if(rcmType && maxDelayNet >= KURAMOTO_HISTORY) {
+			std::ostringstream m;
+			m << "Kuramoto couplings support delays up to " << KURAMOTO_HISTORY - 1 << " cycles; found " << maxDelayNet;
+			throw exception(NEMO_INVALID_INPUT, m.str());
+		}
 		/* neuron groups in type order, sim-local indices contiguous */
 		unsigned base = 0;
 		groupBase.resize(groups.size());
 		simParam.resize(groups.size());
 		simState.resize(groups.size());
+		stateCurrent.assign(groups.size(), 0u);
 		simRng.resize(groups.size());
 		for(size_t gi = 0; gi < groups.size(); ++gi) {
 			NeuronGroup& grp = groups[gi];
@@ -621,7 +743,7 @@ struct Oracle {
 			size_t n = grp.gidx.size();
 			groupBase[gi] = base;
 			simParam[gi].resize(ti.nparam * n);
-			simState[gi].resize(ti.nstate * n);
+			simState[gi].assign(size_t(ti.history) * ti.nstate * n, 0.0f);
 			for(unsigned p = 0; p < ti.nparam; ++p)
 				std::copy(grp.param[p].begin(), grp.param[p].end(), simParam[gi].begin() + p * n);
 			for(unsigned p = 0; p < ti.nstate; ++p)
@@ -714,6 +836,36 @@ struct Oracle {
 				rcmSource[r] = ls[i];
 				rcmDelay[r] = synDelay[i];
 				rcmForward[r] = flatTerm[i];
+			}
+		}
+		if(rcmType) {
+			/* every synapse, per target in the order of the network's synapse
+			 * iterator: sources ascending by global index, each source's
+			 * synapses in insertion order (src/nemo/NetworkImpl.cpp:311-325
+			 * over the std::map of axons; src/nemo/ConnectivityMatrix.cpp:79-87) */
+			std::vector<size_t> order(S);
+			for(size_t i = 0; i < S; ++i) order[i] = i;
+			std::stable_sort(order.begin(), order.end(),
+					[&](size_t a, size_t b) { return synSrc[a] < synSrc[b]; });
+			kStart.assign(N + 1, 0);
+			for(size_t i = 0; i < S; ++i) kStart[lt[i] + 1] += 1;
+			for(size_t n = 0; n < N; ++n) kStart[n + 1] += kStart[n];
+			kSource.resize(S);
+			kDelay.resize(S);
+			kWeight.resize(S);
+			std::vector<size_t> kf(kStart.begin(), kStart.end() - 1);
+			for(size_t k = 0; k < S; ++k) {
+				size_t i = order[k];
+				size_t r = kf[lt[i]]++;
+				kSource[r] = ls[i];
+				kDelay[r] = synDelay[i];
+				kWeight[r] = synWeight[i];
+			}
+			/* src/nemo/cpu/Neurons.cpp:42-46: the type's init after its RNG */
+			for(size_t gi = 0; gi < groups.size(); ++gi) {
+				if(groups[gi].model != MODEL_KURAMOTO || groups[gi].gidx.empty()) continue;
+				size_t n = groups[gi].gidx.size();
+				init_kuramoto(groupBase[gi], unsigned(groupBase[gi] + n), &simParam[gi][0], &simState[gi][0], n);
 			}
 		}
 		firingFifo.assign(1, std::vector<unsigned>());  // dummy entry (FiringBuffer.cpp:7-12)
@@ -810,6 +962,12 @@ struct Oracle {
 						&recentFiring[0], &fired[0]);
 			} else if(model == MODEL_INPUT) {
 				update_input(start, end, &fstim[0], &recentFiring[0], &fired[0]);
+			} else if(model == MODEL_KURAMOTO) {
+				/* src/nemo/cpu/Neurons.cpp:121-139: the current plane moves on
+				 * before the update; the cycle is the timer's (unsigned) */
+				stateCurrent[gi] = unsigned((unsigned(cycle) + 1u) % KURAMOTO_HISTORY);
+				update_kuramoto(start, end, unsigned(cycle), &simParam[gi][0], &simState[gi][0], n,
+						kStart, kSource, kDelay, kWeight);
 			} else {
 				update_if_curr_exp(start, end, model == MODEL_IF_CURR_EXP_NOISE,
 						&simParam[gi][0], n, &simState[gi][0], n,
@@ -846,7 +1004,8 @@ struct Oracle {
 			m << "Invalid state variable index " << var;
 			throw exception(NEMO_INVALID_INPUT, m.str());
 		}
-		return simState[a.first][var * groups[a.first].gidx.size() + a.second];
+		size_t n = groups[a.first].gidx.size();
+		return simState[a.first][(size_t(stateCurrent[a.first]) * ti.nstate + var) * n + a.second];
 	}
 
 	float getParam(unsigned g, unsigned p) const {
@@ -862,12 +1021,14 @@ struct Oracle {
 
 	void setState(unsigned g, unsigned var, float v) {
 		std::pair<unsigned, unsigned> a = groupIndex(g);
-		if(var >= TYPES[groups[a.first].model].nstate) {
+		const TypeInfo& ti = TYPES[groups[a.first].model];
+		if(var >= ti.nstate) {
 			std::ostringstream m;
 			m << "Invalid state variable index " << var;
 			throw exception(NEMO_INVALID_INPUT, m.str());
 		}
-		simState[a.first][var * groups[a.first].gidx.size() + a.second] = v;
+		size_t n = groups[a.first].gidx.size();
+		simState[a.first][(size_t(stateCurrent[a.first]) * ti.nstate + var) * n + a.second] = v;
 	}
 
 	void setParam(unsigned g, unsigned p, float v) {
@@ -889,6 +1050,8 @@ struct Oracle {
 				<< ti.nparam + ti.nstate << ", found " << nargs;
 			throw exception(NEMO_INVALID_INPUT, m.str());
 		}
+		/* src/nemo/cpu/Neurons.cpp:66-81: the state goes to plane 0, whichever
+		 * plane is current */
 		size_t n = groups[a.first].gidx.size();
 		for(unsigned i = 0; i < ti.nparam; ++i) simParam[a.first][i * n + a.second] = args[i];
 		for(unsigned i = 0; i < ti.nstate; ++i) simState[a.first][i * n + a.second] = args[ti.nparam + i];
@@ -1174,6 +1337,12 @@ void oracle_libm_logf_grid(uint32_t first, size_t n, float* out)
 void oracle_libm_sinf(const float* a, size_t n, float* out)
 {
 	for(size_t i = 0; i < n; ++i) out[i] = sinf(a[i]);
+}
+
+/* The Kuramoto phase wrap with the host libm (Kuramoto.cpp:121) */
+void oracle_kuramoto_wrap(const float* a, size_t n, float* out)
+{
+	for(size_t i = 0; i < n; ++i) out[i] = kuramoto_wrap(a[i]);
 }
 
 void oracle_init_rng(unsigned minIdx, unsigned maxIdx, uint32_t* out)

@@ -18,7 +18,8 @@ _c_u8_p = ctypes.POINTER(ctypes.c_ubyte)
 _c_i32_p = ctypes.POINTER(ctypes.c_int32)
 _c_size_p = ctypes.POINTER(ctypes.c_size_t)
 
-MODEL_IDS = {"Izhikevich": 0, "IF_curr_exp": 1, "IF_curr_exp_noise": 2, "PoissonSource": 3, "Input": 4}
+MODEL_IDS = {"Izhikevich": 0, "IF_curr_exp": 1, "IF_curr_exp_noise": 2, "PoissonSource": 3, "Input": 4,
+             "Kuramoto": 5}
 
 _lib = None
 
@@ -79,6 +80,7 @@ def load():
         "oracle_nrand_seq": (None, [_c_uint_p, ctypes.c_size_t, _c_float_p]),
         "oracle_libm_logf_grid": (None, [ctypes.c_uint32, ctypes.c_size_t, _c_float_p]),
         "oracle_libm_sinf": (None, [_c_float_p, ctypes.c_size_t, _c_float_p]),
+        "oracle_kuramoto_wrap": (None, [_c_float_p, ctypes.c_size_t, _c_float_p]),
         "oracle_init_rng": (None, [ctypes.c_uint, ctypes.c_uint, _c_uint_p]),
         "oracle_mt19937_nth": (ctypes.c_uint32, [ctypes.c_uint]),
     }
@@ -368,6 +370,14 @@ def libm_sinf(a):
     a = np.ascontiguousarray(a, np.float32)
     out = np.zeros_like(a)
     load().oracle_libm_sinf(_f(a), len(a), _f(out))
+    return out
+
+
+def kuramoto_wrap(a):
+    """The Kuramoto phase wrap with the host C library's fmodf."""
+    a = np.ascontiguousarray(a, np.float32)
+    out = np.zeros_like(a)
+    load().oracle_kuramoto_wrap(_f(a), len(a), _f(out))
     return out
 
 

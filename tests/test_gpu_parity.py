@@ -644,11 +644,12 @@ def test_c3_60_epochs(nemo_mod, oracle_mod, progress):
 
 @pytest.mark.slow
 @pytest.mark.parametrize("n,m,dmax,stdp_on", [(300000, 100, 20, False), (100000, 100, 64, False),
-                                              (80000, 100, 64, True)])
+                                              (150000, 100, 44, True)])
 def test_beyond_the_fused_range(nemo_mod, oracle_mod, progress, n, m, dmax, stdp_on):
     """Networks the fused kernel cannot hold -- more partitions than can be
     co-resident (300k neurons at delay 20, 100k at delay 64) or an LDS ring
-    too deep for enough partitions (80k at delay 64, with STDP) -- run on the
+    too deep for enough partitions (150k at delay 44 with STDP, whose 20-cycle
+    postfire window brings the history to the limit of 64) -- run on the
     plugin step in the automatic mode: 200 cycles (with applyStdp after 100)
     against the oracle, full raster and a sample of the state and weights."""
     spec = examples.izhikevich_80_20(n, m, seed=61, dmax=dmax, stdp=stdp_on)

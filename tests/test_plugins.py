@@ -26,17 +26,18 @@ def _external_model_dir(tmp_path, name, template="Izhikevich"):
 
 
 BUILTIN = ("Izhikevich", "IF_curr_exp", "IF_curr_exp_noise", "PoissonSource", "Input")
+SHIPPED = BUILTIN + ("Kuramoto",)   # plugins shipped in nemosim_amd/plugins
 
 
 def test_plugin_libraries_built():
-    for m in BUILTIN:
+    for m in SHIPPED:
         assert os.path.exists(os.path.join(PLUGINS, "hip", "lib%s.so" % m))
         assert os.path.exists(os.path.join(PLUGINS, m + ".ini"))
 
 
 def test_plugin_exports_abi_symbols():
     import ctypes
-    for m in BUILTIN:
+    for m in SHIPPED:
         lib = ctypes.CDLL(os.path.join(PLUGINS, "hip", "lib%s.so" % m))
         assert hasattr(lib, "hip_update_neurons") and hasattr(lib, "hip_init_neurons")
 
@@ -184,3 +185,15 @@ def test_external_model_runs_through_its_plugin(nemo_mod, oracle_mod, tmp_path):
     rc, ri = orc.run(300)
     d = first_divergence(rc, ri, gc, gi)
     assert d is None, "first diverging cycle %d: %s" % d
+
+
+def test_descriptor_history_and_rcm_fields(nemo_mod, tmp_path):
+    """history= of a descriptor is read: history=0 (no cycle of state kept)
+    is refused when the type is added."""
+    d = tmp_path / "plugins"
+    (d / "hip").mkdir(parents=True)
+    (d / "ZeroHistory.ini").write_text("parameters=1\nstate-variables=1\nmembrane-potential=0\nhistory=0\n")
+    nemo_mod.add_plugin_path(str(d))
+    net = nemo_mod.Network()
+    with pytest.raises(nemo_mod.NemoError):
+        net.add_neuron_type("ZeroHistory")
