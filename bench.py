@@ -20,7 +20,8 @@ Other workloads (--workload): "stdp" (50k x 1000, STDP, applyStdp every
 1000 cycles), "batch" (256 networks of 1k-20k neurons, batched persistent
 launch vs one plain run per network), "serve" (Poisson requests under an
 HBM budget, p50/p99 latency), "step" (NeMo's per-cycle nemo_step at
-1k x 100 and 200k x 1000).
+1k x 100 and 200k x 1000), "kuramoto" (Kuramoto oscillators through the
+plugin; give --neurons and --synapses, e.g. 50000 and 100).
 """
 import argparse
 import json
@@ -481,6 +482,107 @@ def bench_serve(args):
     }
 
 
+def bench_kuramoto(args):
+    """Kuramoto oscillators through the Kuramoto plugin: N oscillators with m
+    incoming couplings each (random sources, delays U{1..31}, weights
+    N(0, 0.02), frequencies U[0, 0.2)), run in launches of --launch-steps
+    cycles. Metric: coupling terms per second (each cycle evaluates every
+    coupling once in each of the four Runge-Kutta stages). Parity: the
+    oracle runs the same cycles and every oscillator's final phase must
+    match bit for bit; the oracle's own time on the host is the CPU
+    baseline."""
+    import nemosim_amd
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    n, m = args.neurons, args.synapses
+    rng = np.random.default_rng(args.seed)
+    freq = rng.uniform(0.0, 0.2, n).astype(np.float32)
+    phase = rng.uniform(0.0, 2 * np.pi, n).astype(np.float32)
+    rowptr = np.arange(n + 1, dtype=np.uint64) * m
+    tgt = rng.integers(0, n, n * m).astype(np.uint32)
+    dly = rng.integers(1, 32, n * m).astype(np.uint8)
+    w = rng.normal(0.0, 0.02, n * m).astype(np.float32)
+    args2 = np.stack([freq, phase], axis=1)
+    net = nemosim_amd.Network()
+    ty = net.add_neuron_type("Kuramoto")
+    net.add_neurons(ty, np.arange(n, dtype=np.uint32), args2)
+    net.add_synapses_csr(rowptr, tgt, dly, w)
+    t0 = time.perf_counter()
+    sim = nemosim_amd.Simulation(net, nemosim_amd.Configuration())
+    log("kuramoto: %d oscillators x %d couplings built in %.1f s" % (n, m, time.perf_counter() - t0))
+    L = max(1, min(args.launch_steps, args.steps))
+    done = 0
+    while done < args.warmup:
+        k = min(L, args.warmup - done)
+        sim.run(k)
+        done += k
+    wall, kernel_ms, launches, done = 0.0, 0.0, 0, 0
+    while done < args.steps:
+        k = min(L, args.steps - done)
+        t0 = time.perf_counter()
+        sim.run(k)
+        wall += time.perf_counter() - t0
+        kernel_ms += sim.info()["last_kernel_ms"]
+        launches += 1
+        done += k
+    terms = 4.0 * n * m * args.steps
+    got = np.array([sim.get_neuron_state(i, 0) for i in range(n)], np.float32)
+    result = {
+        "metric": "Kuramoto coupling terms/s",
+        "value": terms / wall,
+        "unit": "coupling terms/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "ms_per_sim_second": wall * 1e3 / (args.steps / 1000.0),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (sinf reduction and polynomial in f64)",
+        "data": "synthetic (seeded random oscillator network)",
+        "config": {"workload": "kuramoto", "oscillators": n, "couplings_per_oscillator": m,
+                   "delays": "U{1..31}", "launch_cycles": L},
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "kuramoto_update_kernel + per-cycle plugin-step kernels",
+            # 16 B per coupling per cycle read once (source, delay, weight,
+            # source phase) + 12 B per oscillator (frequency, phase in, phase out)
+            "achieved": (16.0 * n * m + 12.0 * n) * args.steps / (kernel_ms / 1e3) / 1e9,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": (16.0 * n * m + 12.0 * n) * args.steps / (kernel_ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel_ms_per_launch": kernel_ms / launches,
+        },
+        "cpu_baseline": None,
+        "parity": None,
+    }
+    if not args.no_parity:
+        orc = oracle.Oracle()
+        orc.add_neurons("Kuramoto", np.arange(n, dtype=np.uint32), args2)
+        orc.add_synapses_csr(rowptr, tgt, dly, w)
+        orc.finalize()
+        t0 = time.perf_counter()
+        for _ in range(args.warmup + args.steps):
+            orc.step()
+        host = time.perf_counter() - t0
+        ref = orc.state_all(0)
+        bad = int(np.count_nonzero(got.view(np.uint32) != ref.view(np.uint32)))
+        status = ("bit-exact: every oscillator's phase after cycle %d" % (args.warmup + args.steps - 1)
+                  if bad == 0 else "DIVERGED: %d of %d phases differ" % (bad, n))
+        log("parity: " + status)
+        result["parity"] = {"status": status, "oracle_s": host}
+        if not args.no_cpu_baseline:
+            cyc = args.warmup + args.steps
+            result["cpu_baseline"] = {
+                "value": 4.0 * n * m * cyc / host, "unit": "coupling terms/s",
+                "cores": int(oracle.load().oracle_num_threads()), "kind": "port",
+                "sample": "the same network, cycles 0..%d on the oracle (%.1f s)" % (cyc - 1, host),
+                "ms_per_sim_second": host * 1e3 / (cyc / 1000.0)}
+    return result
+
+
 def copy_calibration():
     """A plain device copy of known size (1 GiB read + 1 GiB written per
     launch, 4 launches) for scaling rocprof's FETCH_SIZE / WRITE_SIZE to
@@ -500,7 +602,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", default="plain",
-                    choices=["plain", "stdp", "batch", "serve", "step", "copy-calibration"])
+                    choices=["plain", "stdp", "batch", "serve", "step", "kuramoto", "copy-calibration"])
     ap.add_argument("--steps", type=int, default=5000, help="timed cycles")
     ap.add_argument("--warmup", type=int, default=1000, help="warm-up cycles")
     ap.add_argument("--launch-steps", type=int, default=1000, help="cycles per persistent launch")
@@ -538,6 +640,8 @@ def main():
         result = bench_batch(args)
     elif args.workload == "step":
         result = bench_step(args)
+    elif args.workload == "kuramoto":
+        result = bench_kuramoto(args)
     elif args.workload == "copy-calibration":
         result = copy_calibration()
     else:
