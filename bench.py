@@ -427,6 +427,7 @@ def bench_serve(args):
     from nemosim_amd import autotune
     srv = serving.Server(hbm_budget_bytes=budget, chunk_cycles=args.serve_chunk,
                          build_workers=args.serve_workers, keep_rasters=False,
+                         device_construction=args.serve_device_construction,
                          tuning=autotune.load_table() if args.serve_tuned else None)
     log("serve: %d requests, budget %.0f GiB, chunks of %d cycles" % (len(reqs), args.serve_budget_gb, args.serve_chunk))
     stats = srv.serve(reqs, realtime=True)
@@ -457,6 +458,7 @@ def bench_serve(args):
             % (b["requests"][0], b["requests"][1], b["latency_s_mean"], b["wait_s_mean"], b["build_s_mean"],
                b["run_s_mean"], b["latency_s_p99"]))
     log("  queue depth %s" % depth)
+    log("  GPU busy %.1f s of %.1f s in %d batched chunks" % (stats["gpu_busy_s"], stats["wall_s"], stats["chunks"]))
     return {
         "metric": "synaptic events/s (serving)",
         "value": stats["events_per_s"],
@@ -470,12 +472,14 @@ def bench_serve(args):
         "dtype": "q11.20-i32/f32/f64",
         "data": "synthetic (Poisson 50 req/s, N log-uniform [1k,50k], m U{100..1000}, 30% STDP, 30% IF_curr_exp_noise)",
         "config": {"workload": "serving", "requests": stats["requests"], "budget_gib": args.serve_budget_gb,
-                   "chunk_cycles": args.serve_chunk},
+                   "chunk_cycles": args.serve_chunk, "build_workers": args.serve_workers,
+                   "device_construction": bool(args.serve_device_construction)},
         "p50_latency_s": stats["p50_latency_s"],
         "p99_latency_s": stats["p99_latency_s"],
         "requests_per_s": stats["requests_per_s"],
         "offered_requests_per_s": offered,
         "gpu_busy_s": stats["gpu_busy_s"],
+        "chunks": stats["chunks"],
         "wall_s": stats["wall_s"],
         "latency_by_arrival": by_arrival,
         "queue_depth": depth,
@@ -623,6 +627,8 @@ def main():
     ap.add_argument("--serve-budget-gb", type=float, default=64.0)
     ap.add_argument("--serve-chunk", type=int, default=100)
     ap.add_argument("--serve-workers", type=int, default=8)
+    ap.add_argument("--serve-device-construction", action="store_true",
+                    help="serving: build each network on the device instead of the host")
     ap.add_argument("--serve-tuned", action="store_true", help="serving: partition sizes and delivery lanes "
                     "from the batch tuning table instead of the backend's defaults")
     args = ap.parse_args()

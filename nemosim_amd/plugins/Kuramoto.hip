@@ -13,10 +13,11 @@
  * the host C library's sinf restated in csrc/device/libm_f32.h, fmodf exact),
  * so the phases agree bit for bit with the CPU backend.
  *
- * One lane per oscillator: the four stages of a target depend on each
- * other and each is an ordered sum over its incoming couplings, so the
- * parallelism is across oscillators. Phase history: plane c mod 32 of the
- * state holds the phase of cycle c (the descriptor's history=32).
+ * One wave per oscillator: the four stages of a target depend on each other
+ * and each is an ordered float sum over its incoming couplings, so the
+ * lanes share the loads and the sines of a stage and the sum runs through
+ * them in order. Phase history: plane c mod 32 of the state holds the phase
+ * of cycle c (the descriptor's history=32).
  */
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -28,39 +29,112 @@ namespace {
 
 constexpr unsigned HISTORY = 32u;   /* history= of Kuramoto.ini */
 
-/* sum of w * sin(theta_source - x) over the couplings [r0, r1) of one
- * target, in order (src/nemo/cpu/plugins/Kuramoto.cpp:18-29) */
-__device__ inline float coupling(const uint32_t* src, const uint32_t* dly, const float* w, uint32_t r0, uint32_t r1,
-		const float* state, size_t hstride, unsigned cycle, unsigned start, float x)
+constexpr unsigned CHUNKS = 4u;     /* chunks of 64 couplings a wave holds in registers */
+
+/* s + p(lane 0) + p(lane 1) + ... + p(lane cnt-1), each addition rounded to
+ * float, in lane order: the reference's sequential sum over one chunk of 64
+ * couplings. cnt (1..64) is wave-uniform; every lane ends with the same s. */
+__device__ inline float ordered_sum(float s, float p, unsigned cnt)
 {
-	float sum = 0.0f;
-	for(uint32_t r = r0; r < r1; ++r) {
-		const unsigned plane = (cycle - (dly[r] - 1u)) % HISTORY;
-		const float sp = state[size_t(plane) * hstride + (src[r] - start)];
-		sum += w[r] * nemo_libm::sinf_glibc(sp - x);
+	const int bits = __float_as_int(p);
+	if(cnt == 64u) {
+#pragma unroll
+		for(int k = 0; k < 64; ++k) {
+			s = s + __int_as_float(__builtin_amdgcn_readlane(bits, k));
+		}
+	} else {
+#pragma unroll
+		for(int k = 0; k < 64; ++k) {
+			if(unsigned(k) < cnt) {
+				s = s + __int_as_float(__builtin_amdgcn_readlane(bits, k));
+			}
+		}
 	}
-	return sum;
+	return s;
 }
 
-__global__ void kuramoto_update_kernel(unsigned start, unsigned end, unsigned cycle,
+/* weight and source phase of the couplings rb + c*64 + lane, c < CHUNKS
+ * (zero past rend); the phase of the source at cycle t - (delay - 1) */
+__device__ inline void load_chunks(float (&sp)[CHUNKS], float (&wt)[CHUNKS], uint32_t rb, uint32_t rend,
+		unsigned lane, const uint32_t* src, const uint32_t* dly, const float* w, const float* state,
+		size_t hstride, unsigned cycle, unsigned start)
+{
+#pragma unroll
+	for(unsigned c = 0; c < CHUNKS; ++c) {
+		const uint32_t r = rb + c * 64u + lane;
+		sp[c] = 0.0f;
+		wt[c] = 0.0f;
+		if(r < rend) {
+			const unsigned plane = (cycle - (dly[r] - 1u)) % HISTORY;
+			sp[c] = state[size_t(plane) * hstride + (src[r] - start)];
+			wt[c] = w[r];
+		}
+	}
+}
+
+/* s plus the couplings held in sp/wt (cnt of them from the first chunk on,
+ * at most CHUNKS*64), each w * sin(theta_source - x) rounded to float
+ * (src/nemo/cpu/plugins/Kuramoto.cpp:18-29), summed in order */
+__device__ inline float sum_chunks(float s, const float (&sp)[CHUNKS], const float (&wt)[CHUNKS], float x, uint32_t cnt)
+{
+#pragma unroll
+	for(unsigned c = 0; c < CHUNKS; ++c) {
+		if(c * 64u >= cnt) {
+			break;
+		}
+		const float p = wt[c] * nemo_libm::sinf_glibc(sp[c] - x);
+		s = ordered_sum(s, p, min(64u, cnt - c * 64u));
+	}
+	return s;
+}
+
+/* One wave per oscillator: its lanes load 64 couplings at a time (the row of
+ * the reverse matrix is contiguous, so the loads are coalesced) and evaluate
+ * their sines together; the sum then runs through the lanes in order. Rows
+ * of up to CHUNKS*64 couplings stay in registers over the four stages;
+ * longer rows are read again for each stage. */
+__global__ void __launch_bounds__(256) kuramoto_update_kernel(unsigned start, unsigned end, unsigned cycle,
 		const float* freq, float* state, size_t hstride, const uint32_t* rstart,
 		const uint32_t* src, const uint32_t* dly, const float* w)
 {
-	const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
-	if(i >= end - start) {
+	const unsigned lane = threadIdx.x & 63u;
+	const unsigned o = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);   /* wave-uniform */
+	if(o >= end - start) {
 		return;
 	}
 	const float* phase0 = state + size_t(cycle % HISTORY) * hstride;
 	float* phase1 = state + size_t((cycle + 1u) % HISTORY) * hstride;
-	const float f = freq[i];
-	const float theta = phase0[i];
-	const uint32_t r0 = rstart[start + i], r1 = rstart[start + i + 1u];
-	const float k0 = f + coupling(src, dly, w, r0, r1, state, hstride, cycle, start, theta);
-	const float k1 = f + coupling(src, dly, w, r0, r1, state, hstride, cycle, start, theta + k0 * 0.5f);
-	const float k2 = f + coupling(src, dly, w, r0, r1, state, hstride, cycle, start, theta + k1 * 0.5f);
-	const float k3 = f + coupling(src, dly, w, r0, r1, state, hstride, cycle, start, theta + k2);
-	const float next = theta + (k0 + 2.0f * k1 + 2.0f * k2 + k3) / 6.0f;
-	phase1[i] = nemo_libm::kuramoto_wrap(next);
+	const float f = freq[o];
+	const float theta = phase0[o];
+	const uint32_t r0 = rstart[start + o], r1 = rstart[start + o + 1u];
+	float k0, k1, k2, k3;
+	if(r1 - r0 <= CHUNKS * 64u) {
+		float sp[CHUNKS], wt[CHUNKS];
+		load_chunks(sp, wt, r0, r1, lane, src, dly, w, state, hstride, cycle, start);
+		const uint32_t cnt = r1 - r0;
+		k0 = f + sum_chunks(0.0f, sp, wt, theta, cnt);
+		k1 = f + sum_chunks(0.0f, sp, wt, theta + k0 * 0.5f, cnt);
+		k2 = f + sum_chunks(0.0f, sp, wt, theta + k1 * 0.5f, cnt);
+		k3 = f + sum_chunks(0.0f, sp, wt, theta + k2, cnt);
+	} else {
+		auto stage = [&](float x) {
+			float s = 0.0f;
+			for(uint32_t rb = r0; rb < r1; rb += CHUNKS * 64u) {
+				float sp[CHUNKS], wt[CHUNKS];
+				load_chunks(sp, wt, rb, r1, lane, src, dly, w, state, hstride, cycle, start);
+				s = sum_chunks(s, sp, wt, x, r1 - rb);
+			}
+			return s;
+		};
+		k0 = f + stage(theta);
+		k1 = f + stage(theta + k0 * 0.5f);
+		k2 = f + stage(theta + k1 * 0.5f);
+		k3 = f + stage(theta + k2);
+	}
+	if(lane == 0) {
+		const float next = theta + (k0 + 2.0f * k1 + 2.0f * k2 + k3) / 6.0f;
+		phase1[o] = nemo_libm::kuramoto_wrap(next);
+	}
 }
 
 /* src/nemo/cpu/plugins/Kuramoto.cpp:133-152: the planes of cycles -1 .. -31
@@ -95,7 +169,7 @@ int hip_update_neurons(void* stream, unsigned cycle, unsigned start, unsigned en
 		return int(hipErrorInvalidValue);   /* the descriptor asks for sources, delays and weights */
 	}
 	const unsigned n = end - start;
-	hipLaunchKernelGGL(kuramoto_update_kernel, dim3((n + 255u) / 256u), dim3(256), 0,
+	hipLaunchKernelGGL(kuramoto_update_kernel, dim3((n + 3u) / 4u), dim3(256), 0,
 			static_cast<hipStream_t>(stream), start, end, cycle, param, state, stateHistoryStride,
 			rcm->start, rcm->source, rcm->delay, rcm->weight);
 	return int(hipGetLastError());

@@ -146,6 +146,7 @@ class Server(object):
         self.used = 0
         self.lock = threading.Lock()
         self.gpu_busy_s = 0.0
+        self.chunks = 0       # batched launches of the resident set
         self.depth = []       # (time, waiting, building, resident) at each pass of the loop
 
     def _transient(self, r):
@@ -165,6 +166,7 @@ class Server(object):
         t0 = time.perf_counter()
         ev = nemo_api.run_batch(sims, steps)
         self.gpu_busy_s += time.perf_counter() - t0
+        self.chunks += 1
         for r, k, e in zip(active, steps, ev):
             r.events += int(e)
             if self.keep_rasters:
@@ -242,6 +244,7 @@ class Server(object):
             "requests": len(done),
             "wall_s": wall,
             "gpu_busy_s": self.gpu_busy_s,
+            "chunks": self.chunks,
             "p50_latency_s": float(np.percentile(lat, 50)) if len(lat) else None,
             "p99_latency_s": float(np.percentile(lat, 99)) if len(lat) else None,
             "synaptic_events": int(events),
