@@ -422,7 +422,7 @@ def bench_batch(args):
 def bench_serve(args):
     import nemosim_amd  # noqa: F401
     from nemosim_amd import serving
-    reqs = serving.poisson_requests(args.serve_requests, rate_hz=50.0, seed=args.seed)
+    reqs = serving.poisson_requests(args.serve_requests, rate_hz=args.serve_rate, seed=args.seed)
     budget = int(args.serve_budget_gb * (1 << 30))
     from nemosim_amd import autotune
     srv = serving.Server(hbm_budget_bytes=budget, chunk_cycles=args.serve_chunk,
@@ -470,7 +470,8 @@ def bench_serve(args):
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "q11.20-i32/f32/f64",
-        "data": "synthetic (Poisson 50 req/s, N log-uniform [1k,50k], m U{100..1000}, 30% STDP, 30% IF_curr_exp_noise)",
+        "data": "synthetic (Poisson %g req/s, N log-uniform [1k,50k], m U{100..1000}, 30%% STDP, "
+                "30%% IF_curr_exp_noise)" % args.serve_rate,
         "config": {"workload": "serving", "requests": stats["requests"], "budget_gib": args.serve_budget_gb,
                    "chunk_cycles": args.serve_chunk, "build_workers": args.serve_workers,
                    "device_construction": bool(args.serve_device_construction)},
@@ -627,6 +628,7 @@ def main():
     ap.add_argument("--serve-budget-gb", type=float, default=64.0)
     ap.add_argument("--serve-chunk", type=int, default=100)
     ap.add_argument("--serve-workers", type=int, default=8)
+    ap.add_argument("--serve-rate", type=float, default=50.0, help="serving: Poisson arrival rate (requests/s)")
     ap.add_argument("--serve-device-construction", action="store_true",
                     help="serving: build each network on the device instead of the host")
     ap.add_argument("--serve-tuned", action="store_true", help="serving: partition sizes and delivery lanes "

@@ -16,7 +16,7 @@
  * One wave per oscillator: the four stages of a target depend on each other
  * and each is an ordered float sum over its incoming couplings, so the
  * lanes share the loads and the sines of a stage and the sum runs through
- * them in order. Phase history: plane c mod 32 of the state holds the phase
+ * them in order (through LDS, as 64 dependent register adds). Phase history: plane c mod 32 of the state holds the phase
  * of cycle c (the descriptor's history=32).
  */
 #include <hip/hip_runtime.h>
@@ -33,23 +33,38 @@ constexpr unsigned CHUNKS = 4u;     /* chunks of 64 couplings a wave holds in re
 
 /* s + p(lane 0) + p(lane 1) + ... + p(lane cnt-1), each addition rounded to
  * float, in lane order: the reference's sequential sum over one chunk of 64
- * couplings. cnt (1..64) is wave-uniform; every lane ends with the same s. */
-__device__ inline float ordered_sum(float s, float p, unsigned cnt)
+ * couplings. The products go through the wave's 64 floats of LDS (`buf`)
+ * and every lane reads them all back (broadcast reads, four at a time), so
+ * the chain is 64 dependent adds on registers. cnt (1..64) is wave-uniform;
+ * every lane ends with the same s. A wave's LDS operations complete in
+ * order, so no barrier is needed between the store and the loads. */
+__device__ inline float ordered_sum(float s, float p, unsigned cnt, float* buf, unsigned lane)
 {
-	const int bits = __float_as_int(p);
+	buf[lane] = p;
+	__builtin_amdgcn_wave_barrier();
+	const float4* b4 = reinterpret_cast<const float4*>(buf);
 	if(cnt == 64u) {
 #pragma unroll
-		for(int k = 0; k < 64; ++k) {
-			s = s + __int_as_float(__builtin_amdgcn_readlane(bits, k));
+		for(int q = 0; q < 16; ++q) {
+			const float4 v = b4[q];
+			s = s + v.x;
+			s = s + v.y;
+			s = s + v.z;
+			s = s + v.w;
 		}
 	} else {
 #pragma unroll
-		for(int k = 0; k < 64; ++k) {
-			if(unsigned(k) < cnt) {
-				s = s + __int_as_float(__builtin_amdgcn_readlane(bits, k));
+		for(int q = 0; q < 16; ++q) {
+			if(unsigned(4 * q) < cnt) {
+				const float4 v = b4[q];
+				s = s + v.x;
+				if(unsigned(4 * q + 1) < cnt) s = s + v.y;
+				if(unsigned(4 * q + 2) < cnt) s = s + v.z;
+				if(unsigned(4 * q + 3) < cnt) s = s + v.w;
 			}
 		}
 	}
+	__builtin_amdgcn_wave_barrier();   /* the reads are done before the next chunk's store */
 	return s;
 }
 
@@ -75,7 +90,8 @@ __device__ inline void load_chunks(float (&sp)[CHUNKS], float (&wt)[CHUNKS], uin
 /* s plus the couplings held in sp/wt (cnt of them from the first chunk on,
  * at most CHUNKS*64), each w * sin(theta_source - x) rounded to float
  * (src/nemo/cpu/plugins/Kuramoto.cpp:18-29), summed in order */
-__device__ inline float sum_chunks(float s, const float (&sp)[CHUNKS], const float (&wt)[CHUNKS], float x, uint32_t cnt)
+__device__ inline float sum_chunks(float s, const float (&sp)[CHUNKS], const float (&wt)[CHUNKS], float x, uint32_t cnt,
+		float* buf, unsigned lane)
 {
 #pragma unroll
 	for(unsigned c = 0; c < CHUNKS; ++c) {
@@ -83,7 +99,7 @@ __device__ inline float sum_chunks(float s, const float (&sp)[CHUNKS], const flo
 			break;
 		}
 		const float p = wt[c] * nemo_libm::sinf_glibc(sp[c] - x);
-		s = ordered_sum(s, p, min(64u, cnt - c * 64u));
+		s = ordered_sum(s, p, min(64u, cnt - c * 64u), buf, lane);
 	}
 	return s;
 }
@@ -97,7 +113,9 @@ __global__ void __launch_bounds__(256) kuramoto_update_kernel(unsigned start, un
 		const float* freq, float* state, size_t hstride, const uint32_t* rstart,
 		const uint32_t* src, const uint32_t* dly, const float* w)
 {
+	__shared__ __align__(16) float sumBuf[4][64];   /* one row per wave of the 256-thread block */
 	const unsigned lane = threadIdx.x & 63u;
+	float* buf = sumBuf[threadIdx.x >> 6];
 	const unsigned o = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);   /* wave-uniform */
 	if(o >= end - start) {
 		return;
@@ -112,17 +130,17 @@ __global__ void __launch_bounds__(256) kuramoto_update_kernel(unsigned start, un
 		float sp[CHUNKS], wt[CHUNKS];
 		load_chunks(sp, wt, r0, r1, lane, src, dly, w, state, hstride, cycle, start);
 		const uint32_t cnt = r1 - r0;
-		k0 = f + sum_chunks(0.0f, sp, wt, theta, cnt);
-		k1 = f + sum_chunks(0.0f, sp, wt, theta + k0 * 0.5f, cnt);
-		k2 = f + sum_chunks(0.0f, sp, wt, theta + k1 * 0.5f, cnt);
-		k3 = f + sum_chunks(0.0f, sp, wt, theta + k2, cnt);
+		k0 = f + sum_chunks(0.0f, sp, wt, theta, cnt, buf, lane);
+		k1 = f + sum_chunks(0.0f, sp, wt, theta + k0 * 0.5f, cnt, buf, lane);
+		k2 = f + sum_chunks(0.0f, sp, wt, theta + k1 * 0.5f, cnt, buf, lane);
+		k3 = f + sum_chunks(0.0f, sp, wt, theta + k2, cnt, buf, lane);
 	} else {
 		auto stage = [&](float x) {
 			float s = 0.0f;
 			for(uint32_t rb = r0; rb < r1; rb += CHUNKS * 64u) {
 				float sp[CHUNKS], wt[CHUNKS];
 				load_chunks(sp, wt, rb, r1, lane, src, dly, w, state, hstride, cycle, start);
-				s = sum_chunks(s, sp, wt, x, r1 - rb);
+				s = sum_chunks(s, sp, wt, x, r1 - rb, buf, lane);
 			}
 			return s;
 		};
