@@ -393,10 +393,14 @@ def bench_batch(args):
     ev_batch = int(np.sum(nemosim_amd.run_batch(batched, [steps] * count)))
     wall_batch = time.perf_counter() - t0
     exact = 0
-    for a, b in zip(plain, batched):
+    abytes = 0
+    for spec, a, b in zip(specs, plain, batched):
         ca, ia = a.read_fired()
         cb, ib = b.read_fired()
         exact += int(np.array_equal(ca, cb) and np.array_equal(ia, ib))
+        deg, rows = delay_tables(spec)
+        abytes += algorithmic_bytes(ib.astype(np.int64), deg, rows, spec.n, steps)[0]
+    achieved = abytes / wall_batch / 1e9
     return {
         "metric": "synaptic events/s (batched vs plain)",
         "value": ev_batch / wall_batch,
@@ -416,6 +420,21 @@ def bench_batch(args):
         "wall_plain_s": wall_plain,
         "wall_batched_s": wall_batch,
         "rasters_identical": "%d/%d" % (exact, count),
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "nemo_step_kernel (batched launches)",
+            # the algorithmic bytes of every network (8 B per delivered
+            # record, 8 B per delivered row, 68 B per neuron and cycle) over
+            # the wall time of the whole batched run: launches of several
+            # groups of networks and the host work between them included
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "algorithmic_bytes": abytes,
+            "time_basis": "wall of the batched run",
+        },
     }
 
 

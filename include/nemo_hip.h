@@ -203,11 +203,12 @@ nemo_status_t nemo_hip_get_info(nemo_simulation_t sim, nemo_hip_info_t* info);
  * failure; this gives its status code). */
 nemo_status_t nemo_hip_last_status(void);
 
-/* Evaluate the device's logf / sinf (kind 0 / 1, as the RNG and the
- * Kuramoto model use them) or the Kuramoto phase wrap (kind 2: fmodf by
- * 2*pi, plus 2*pi in double for a negative phase) on n host floats, for
- * checking them against the host C library. With on_device = 0 the same
- * code runs on the host CPU (no GPU needed). */
+/* Evaluate the device's math on n host floats, for checking it against the
+ * host C library: kind 0 = logf and 1 = sinf as the RNG uses them (for the
+ * RNG's arguments), 2 = the Kuramoto phase wrap (fmodf by 2*pi, plus 2*pi in
+ * double for a negative phase), 3 = sinf for any argument (the Kuramoto
+ * model's). With on_device = 0 the same code runs on the host CPU (no GPU
+ * needed). */
 nemo_status_t nemo_hip_eval_libm(int kind, const float in[], float out[], size_t n, int on_device);
 
 /* The Gaussian sequence nrand() yields from RNG state state4, computed on

@@ -11,16 +11,18 @@
  * double precision so they run on gfx950 FP64 and on the host alike.
  *
  * On FMA-capable x86-64 hosts glibc runs an FMA build of these sources, in
- * which the compiler fuses every product whose only use is a sum. sinf is
- * restated in that form (fma() in the same places: the reduction
- * x - n*pi/2 and each multiply-add of the polynomials), because beyond the
- * RNG's inputs the unfused form differs in about 1 of 10^7 arguments
- * (near multiples of pi/2 of magnitude ~100, where the reduction cancels).
- * logf keeps the unfused form, which equals the host's on every input it is
- * used on. Checked against the host libm on every input the RNG can produce
- * -- logf at all x = m * 2^-24 (m = 0..2^24) and sinf at all
- * a = float(u) * 1.4629180792671596810513378043098e-9f, u a 32-bit unsigned
- * -- with zero differences; tests/test_libm_f32.py repeats that check.
+ * which the compiler fuses every product whose only use is a sum. On the
+ * RNG's arguments the fused and unfused forms give the same results: logf
+ * at all x = m * 2^-24 (m = 0..2^24) and sinf at all
+ * a = float(u) * 1.4629180792671596810513378043098e-9f, u a 32-bit unsigned,
+ * were compared with the host libm with zero differences
+ * (tests/test_libm_f32.py repeats that check). The RNG uses the unfused form
+ * (sinf_rng), which needs fewer registers in the step kernel. Beyond those
+ * arguments the unfused form differs from the host in about 1 of 10^7
+ * (near multiples of pi/2 of magnitude ~100, where the reduction cancels),
+ * so sinf_glibc, used by the Kuramoto model on arbitrary phase differences,
+ * is restated in the fused form: fma() in the same places as the host
+ * build (the reduction x - n*pi/2 and each multiply-add of the polynomials).
  *
  * logf is supported for x in [0, 1] only (no NaN / negative paths). sinf
  * takes every float: besides the RNG's arguments, the Kuramoto model feeds
@@ -123,6 +125,54 @@ NEMO_HD inline float sinf_poly(double x, double x2, const SincosfSet& p, int n)
 		double c = fma(x4, p.c2, c1);
 		return float(fma(x6, c2, c));
 	}
+}
+
+/* The polynomials without fused multiply-adds */
+NEMO_HD inline float sinf_poly_unfused(double x, double x2, const SincosfSet& p, int n)
+{
+	if((n & 1) == 0) {
+		double x3 = x * x2;
+		double s1 = p.s2 + x2 * p.s3;
+		double x7 = x3 * x2;
+		double s = x + x3 * p.s1;
+		return float(s + x7 * s1);
+	} else {
+		double x4 = x2 * x2;
+		double c2 = p.c3 + x2 * p.c4;
+		double c1 = p.c0 + x2 * p.c1;
+		double x6 = x4 * x2;
+		double c = c1 + x4 * p.c2;
+		return float(c + x6 * c2);
+	}
+}
+
+/* sinf for the RNG's arguments a in [0, 2*pi + 1ulp] (unfused form):
+ * |a| < pi/4 directly, otherwise one reduction step by pi/2 (n rounded
+ * through a 2^24-scaled integer). In quadrants 2 and 3 glibc evaluates the
+ * cosine polynomial with every coefficient negated, which equals the
+ * negated result (negation is exact in each product and sum). */
+NEMO_HD inline float sinf_rng(float y)
+{
+	const SincosfSet P0 = { 0x1p0, -0x1.ffffffd0c621cp-2, 0x1.55553e1068f19p-5,
+		-0x1.6c087e89a359dp-10, 0x1.99343027bf8c3p-16,
+		-0x1.555545995a603p-3, 0x1.1107605230bc4p-7, -0x1.994eb3774cf24p-13 };
+	const double HPI_INV = 0x1.45F306DC9C883p+23;
+	const double HPI = 0x1.921FB54442D18p0;
+
+	uint32_t top = (f2u(y) >> 20) & 0x7ffu;
+	double x = y;
+	if(top < ((f2u(0x1.921FB6p-1f) >> 20) & 0x7ffu)) {
+		if(top < ((f2u(0x1p-12f) >> 20) & 0x7ffu)) {
+			return y;
+		}
+		return sinf_poly_unfused(x, x * x, P0, 0);
+	}
+	double r = x * HPI_INV;
+	int n = (int32_t(r) + 0x800000) >> 24;
+	x = x - double(n) * HPI;
+	double s = ((n & 3) == 1 || (n & 3) == 2) ? -1.0 : 1.0;
+	const float v = sinf_poly_unfused(x * s, x * x, P0, n);
+	return ((n & 3) == 3) ? -v : v;
 }
 
 /* Bits of 2/pi: entry i is floor(2/pi * 2^(8(i+1))) mod 2^32, i.e. the

@@ -82,7 +82,7 @@ NEMO_HD inline float nrand(Rng& r)
 	float b = float(urand(r)) * 0.00000000023283064365386962890625f;
 	float l = nemo_libm::logf_glibc(1.0f - b);
 	float rr = sqrt_f32(-2.0f * l);
-	return nemo_libm::sinf_glibc(a) * rr;
+	return nemo_libm::sinf_rng(a) * rr;
 }
 
 /* Q11.20 accumulator to float with saturation, src/nemo/fixedpoint.cpp:41-49.

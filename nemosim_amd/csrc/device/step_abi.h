@@ -140,7 +140,7 @@ int nemo_dev_apply_stdp(uint32_t* syn, const uint32_t* rcm_fwd, int32_t* rcm_acc
 int nemo_dev_fill_u32(uint32_t* p, uint32_t value, size_t n, void* stream);
 int nemo_dev_fill_u64(uint64_t* p, uint64_t value, size_t n, void* stream);
 /* Evaluates the device math pieces on inputs in[] (for tests):
- * kind 0 = logf_glibc(in), 1 = sinf_glibc(in), 2 = kuramoto_wrap(in) */
+ * kind 0 = logf_glibc(in), 1 = sinf_rng(in), 2 = kuramoto_wrap(in), 3 = sinf_glibc(in) */
 int nemo_dev_eval_libm(int kind, const float* in_dev, float* out_dev, size_t n, void* stream);
 int nemo_dev_nrand_seq(const uint32_t* state4_dev, float* out_dev, size_t n, void* stream);
 

@@ -1406,7 +1406,8 @@ __global__ void nemo_eval_libm_kernel(int kind, const float* in, float* out, siz
 {
 	for(size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
 		out[i] = (kind == 0) ? nemo_libm::logf_glibc(in[i])
-			: (kind == 1) ? nemo_libm::sinf_glibc(in[i]) : nemo_libm::kuramoto_wrap(in[i]);
+			: (kind == 1) ? nemo_libm::sinf_rng(in[i])
+			: (kind == 2) ? nemo_libm::kuramoto_wrap(in[i]) : nemo_libm::sinf_glibc(in[i]);
 	}
 }
 

@@ -442,13 +442,15 @@ nemo_status_t
 nemo_hip_eval_libm(int kind, const float in[], float out[], size_t n, int on_device)
 {
 	RETURN_CALL(
-		if(kind < 0 || kind > 2) {
-			throw nemo::exception(NEMO_INVALID_INPUT, "kind must be 0 (logf), 1 (sinf) or 2 (Kuramoto phase wrap)");
+		if(kind < 0 || kind > 3) {
+			throw nemo::exception(NEMO_INVALID_INPUT,
+					"kind must be 0 (logf), 1 (the RNG's sinf), 2 (Kuramoto phase wrap) or 3 (sinf, any argument)");
 		}
 		if(!on_device) {
 			for(size_t i = 0; i < n; ++i) {
 				out[i] = kind == 0 ? nemo_libm::logf_glibc(in[i])
-					: kind == 1 ? nemo_libm::sinf_glibc(in[i]) : nemo_libm::kuramoto_wrap(in[i]);
+					: kind == 1 ? nemo_libm::sinf_rng(in[i])
+					: kind == 2 ? nemo_libm::kuramoto_wrap(in[i]) : nemo_libm::sinf_glibc(in[i]);
 			}
 		} else {
 			nemo::hip::DevBuf din, dout;
