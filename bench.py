@@ -477,7 +477,8 @@ def bench_serve(args):
             % (b["requests"][0], b["requests"][1], b["latency_s_mean"], b["wait_s_mean"], b["build_s_mean"],
                b["run_s_mean"], b["latency_s_p99"]))
     log("  queue depth %s" % depth)
-    log("  GPU busy %.1f s of %.1f s in %d batched chunks" % (stats["gpu_busy_s"], stats["wall_s"], stats["chunks"]))
+    log("  GPU busy %.1f s of %.1f s in %d batched chunks; applyStdp %.1f s, releasing finished simulations %.1f s"
+        % (stats["gpu_busy_s"], stats["wall_s"], stats["chunks"], stats["apply_stdp_s"], stats["release_s"]))
     return {
         "metric": "synaptic events/s (serving)",
         "value": stats["events_per_s"],

@@ -147,6 +147,8 @@ class Server(object):
         self.lock = threading.Lock()
         self.gpu_busy_s = 0.0
         self.chunks = 0       # batched launches of the resident set
+        self.apply_s = 0.0    # time in applyStdp calls
+        self.release_s = 0.0  # time releasing finished simulations
         self.depth = []       # (time, waiting, building, resident) at each pass of the loop
 
     def _transient(self, r):
@@ -175,7 +177,9 @@ class Server(object):
                 r.ids.append(ids.copy())
             r.cycles_done += k
             if r.stdp and r.cycles_done % r.stdp_period == 0:
+                t1 = time.perf_counter()
                 r.sim.apply_stdp(r.reward)
+                self.apply_s += time.perf_counter() - t1
 
     def serve(self, requests, realtime=True):
         """Run all requests to completion. With realtime, arrivals follow
@@ -228,7 +232,9 @@ class Server(object):
                 for r in finished:
                     r.done_s = now()
                     if not self.keep_finished:
+                        t1 = time.perf_counter()
                         r.sim = None
+                        self.release_s += time.perf_counter() - t1
                     self.used -= r.estimated_device_bytes()
                     done.append(r)
                 active = [r for r in active if r.cycles_done < r.duration_ms]
@@ -245,6 +251,8 @@ class Server(object):
             "wall_s": wall,
             "gpu_busy_s": self.gpu_busy_s,
             "chunks": self.chunks,
+            "apply_stdp_s": self.apply_s,
+            "release_s": self.release_s,
             "p50_latency_s": float(np.percentile(lat, 50)) if len(lat) else None,
             "p99_latency_s": float(np.percentile(lat, 99)) if len(lat) else None,
             "synaptic_events": int(events),
