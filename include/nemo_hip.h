@@ -203,6 +203,16 @@ nemo_status_t nemo_hip_get_info(nemo_simulation_t sim, nemo_hip_info_t* info);
  * failure; this gives its status code). */
 nemo_status_t nemo_hip_last_status(void);
 
+/* nemo_apply_stdp(sims[i], rewards[i]) for i < n, under one hold of the
+ * device lock, the kernels of all simulations queued before any is waited
+ * for (a serving loop applies STDP to many simulations between chunks).
+ * Each simulation's result equals its own nemo_apply_stdp. On an error
+ * (a simulation without STDP, a reward that rounds to zero) the
+ * simulations before the failing one have been updated, the others not.
+ * No counterpart in NeMo (the batched form of nemo_apply_stdp,
+ * src/nemo.h:133). */
+nemo_status_t nemo_hip_apply_stdp_batch(nemo_simulation_t sims[], size_t n, const float rewards[]);
+
 /* Evaluate the device's math on n host floats, for checking it against the
  * host C library: kind 0 = logf and 1 = sinf as the RNG uses them (for the
  * RNG's arguments), 2 = the Kuramoto phase wrap (fmodf by 2*pi, plus 2*pi in

@@ -144,6 +144,7 @@ SIGNATURES = {
     "nemo_hip_get_info": (ctypes.c_int, [c_void_p, ctypes.POINTER(InfoT)]),
     "nemo_hip_last_status": (ctypes.c_int, []),
     "nemo_hip_eval_libm": (ctypes.c_int, [ctypes.c_int, c_float_p, c_float_p, ctypes.c_size_t, ctypes.c_int]),
+    "nemo_hip_apply_stdp_batch": (ctypes.c_int, [ctypes.POINTER(c_void_p), ctypes.c_size_t, c_float_p]),
     "nemo_hip_nrand_seq": (ctypes.c_int, [c_u32_p, c_float_p, ctypes.c_size_t]),
 }
 

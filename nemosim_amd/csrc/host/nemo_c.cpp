@@ -310,6 +310,27 @@ nemo_step(nemo_simulation_t s, unsigned fstim[], size_t nf, unsigned istim_idx[]
 nemo_status_t nemo_apply_stdp(nemo_simulation_t s, float reward) { RETURN_CALL(requireSim(s); s->sim->applyStdp(reward)); }
 
 nemo_status_t
+nemo_hip_apply_stdp_batch(nemo_simulation_t sims[], size_t n, const float rewards[])
+{
+	RETURN_CALL(
+		if(n == 0) return NEMO_OK;
+		if(!sims || !rewards) throw nemo::exception(NEMO_INVALID_INPUT, "null simulation or reward array");
+		for(size_t i = 0; i < n; ++i) requireSim(sims[i]);
+		std::lock_guard<nemo::DeviceMutex> devLock(nemo::deviceMutex());
+		/* all launches first, then all waits: the kernels of different
+		 * simulations run on their own streams at once */
+		size_t launched = 0;
+		try {
+			for(; launched < n; ++launched) sims[launched]->sim->applyStdpLaunch(rewards[launched]);
+		} catch(...) {
+			for(size_t i = 0; i < launched; ++i) sims[i]->sim->applyStdpFinish();
+			throw;
+		}
+		for(size_t i = 0; i < n; ++i) sims[i]->sim->applyStdpFinish();
+	);
+}
+
+nemo_status_t
 nemo_hip_run(nemo_simulation_t s, unsigned nsteps, const nemo_schedule_t* sched, unsigned long long* events)
 {
 	RETURN_CALL(requireSim(s); s->sim->run(nsteps, sched, events));

@@ -885,7 +885,10 @@ Simulation::loadPlugins(const NetworkImpl& net)
 
 Simulation::~Simulation()
 {
-	std::lock_guard<DeviceMutex> devLock(deviceMutex());
+	/* Without the device lock: nothing here dispatches a kernel (the lock
+	 * keeps kernels away from the start of a persistent launch), and a
+	 * release must not queue behind the uploads of simulations being built.
+	 * The buffers are freed by their destructors after this body. */
 	if(m_stream) {
 		(void) hipStreamSynchronize(static_cast<hipStream_t>(m_stream));
 		(void) hipStreamDestroy(static_cast<hipStream_t>(m_stream));
@@ -1499,6 +1502,13 @@ void
 Simulation::applyStdp(float reward)
 {
 	std::lock_guard<DeviceMutex> devLock(deviceMutex());
+	applyStdpLaunch(reward);
+	applyStdpFinish();
+}
+
+void
+Simulation::applyStdpLaunch(float reward)
+{
 	if(!m_stdp) {
 		throw exception(NEMO_LOGIC_ERROR, "applyStdp called, but no STDP model specified");
 	}
@@ -1511,7 +1521,12 @@ Simulation::applyStdp(float reward)
 	hipStream_t st = static_cast<hipStream_t>(m_stream);
 	hipCheck(nemo_dev_apply_stdp(d_syn.as<uint32_t>(), d_rcmFwd.as<uint32_t>(), d_rcmAcc.as<int32_t>(),
 				m_nplastic, d_stdp.ptr, fxr, st), "applyStdp kernel");
-	hipCheck(hipStreamSynchronize(st), "hipStreamSynchronize");
+}
+
+void
+Simulation::applyStdpFinish()
+{
+	hipCheck(hipStreamSynchronize(static_cast<hipStream_t>(m_stream)), "hipStreamSynchronize");
 	/* spikes in flight now arrive with the new weights */
 	m_replay = true;
 }

@@ -71,6 +71,11 @@ class Simulation
 				const unsigned* istimIdx, const float* istimVal, size_t nistim);
 
 		void applyStdp(float reward);
+		/* applyStdp in two halves, for applying to several simulations
+		 * under one hold of the device lock (the caller holds it): the
+		 * checks and the kernel launch, then the wait for it */
+		void applyStdpLaunch(float reward);
+		void applyStdpFinish();
 
 		/* multi-cycle runs */
 		void run(unsigned nsteps, const nemo_schedule_t* schedule, unsigned long long* events);

@@ -405,6 +405,18 @@ def run_batch(sims, nsteps):
     return ev
 
 
+def apply_stdp_batch(sims, rewards):
+    """apply_stdp(rewards[i]) on every sims[i] under one hold of the device
+    lock (nemo_hip_apply_stdp_batch)."""
+    lib = _lib_checked()
+    n = len(sims)
+    if n == 0:
+        return
+    handles = (ctypes.c_void_p * n)(*[s._h for s in sims])
+    r = np.ascontiguousarray(rewards, dtype=np.float32)
+    _check(lib.nemo_hip_apply_stdp_batch(handles, n, _fptr(r)))
+
+
 def device_count():
     lib = _lib_checked()
     c = ctypes.c_uint()

@@ -143,6 +143,9 @@ class Server(object):
         self.chunk = int(chunk_cycles)
         self.keep_rasters = keep_rasters
         self.pool = concurrent.futures.ThreadPoolExecutor(max_workers=build_workers)
+        # finished simulations are released on this thread: freeing device
+        # memory waits for the device, which the serving loop should not
+        self.reaper = concurrent.futures.ThreadPoolExecutor(max_workers=1)
         self.used = 0
         self.lock = threading.Lock()
         self.gpu_busy_s = 0.0
@@ -169,6 +172,7 @@ class Server(object):
         ev = nemo_api.run_batch(sims, steps)
         self.gpu_busy_s += time.perf_counter() - t0
         self.chunks += 1
+        due = []
         for r, k, e in zip(active, steps, ev):
             r.events += int(e)
             if self.keep_rasters:
@@ -177,9 +181,12 @@ class Server(object):
                 r.ids.append(ids.copy())
             r.cycles_done += k
             if r.stdp and r.cycles_done % r.stdp_period == 0:
-                t1 = time.perf_counter()
-                r.sim.apply_stdp(r.reward)
-                self.apply_s += time.perf_counter() - t1
+                due.append(r)
+        if due:
+            # every applyStdp due at this boundary under one hold of the device lock
+            t1 = time.perf_counter()
+            nemo_api.apply_stdp_batch([r.sim for r in due], [r.reward for r in due])
+            self.apply_s += time.perf_counter() - t1
 
     def serve(self, requests, realtime=True):
         """Run all requests to completion. With realtime, arrivals follow
@@ -233,6 +240,7 @@ class Server(object):
                     r.done_s = now()
                     if not self.keep_finished:
                         t1 = time.perf_counter()
+                        self.reaper.submit(_release, [r.sim])
                         r.sim = None
                         self.release_s += time.perf_counter() - t1
                     self.used -= r.estimated_device_bytes()
@@ -243,6 +251,7 @@ class Server(object):
                                         return_when=concurrent.futures.FIRST_COMPLETED)
             elif i < len(pending):
                 time.sleep(max(0.0, min(0.01, pending[i].arrival_s - now())))
+        self.reaper.submit(lambda: None).result()   # every release done
         wall = now()
         lat = np.array([r.latency_s for r in done])
         events = sum(r.events for r in done)
@@ -260,6 +269,12 @@ class Server(object):
             "sim_cycles": int(sum(r.duration_ms for r in done)),
             "requests_per_s": len(done) / wall if wall > 0 else None,
         }
+
+
+def _release(holder):
+    """Drop the last reference to a simulation (holder: a one-element
+    list), so that it is destroyed on the calling thread."""
+    holder.pop()
 
 
 def raster_of(req):

@@ -394,6 +394,34 @@ def test_batch_equals_alone(nemo_mod, oracle_mod):
         assert d is None, "network %d: first diverging cycle %d: %s" % ((s.n,) + d)
 
 
+def test_apply_stdp_batch_equals_apply_stdp(nemo_mod, oracle_mod):
+    """nemo_hip_apply_stdp_batch gives each simulation what its own
+    nemo_apply_stdp gives (the oracle's weights and raster after it), and
+    refuses a simulation without STDP."""
+    pre, post = examples.stdp_window()
+    stdp = (pre, post, 0.0, 1.0, 0.0, -1.0)
+    specs = [examples.izhikevich_80_20(n, 100, seed=300 + i, stdp=True) for i, n in enumerate([1200, 3000, 2000])]
+    sims = [hip_sim(nemo_mod, s, stdp=stdp) for s in specs]
+    orcs = [oracle_mod.from_spec(s, stdp=stdp) for s in specs]
+    rewards = [1.0, 0.5, 0.8]
+    for sim in sims:
+        sim.run(500)
+    nemo_mod.apply_stdp_batch(sims, rewards)
+    for sim in sims:
+        sim.run(300)
+    for s, sim, orc, r in zip(specs, sims, orcs, rewards):
+        orc.run(500)
+        orc.apply_stdp(r)
+        rc, ri = orc.run(300)
+        gc, gi = sim.read_fired()
+        d = first_divergence(rc, ri, gc, gi)
+        assert d is None, "network %d: first diverging cycle %d: %s" % ((s.n,) + d)
+        compare_weights(sim, orc, s, 2000, seed=1)
+    plain = hip_sim(nemo_mod, examples.izhikevich_80_20(500, 50, seed=1))
+    with pytest.raises(nemo_mod.NemoError):
+        nemo_mod.apply_stdp_batch([sims[0], plain], [1.0, 1.0])
+
+
 @pytest.mark.parametrize("dense", [False, True])
 def test_deferred_and_immediate_delivery(nemo_mod, oracle_mod, dense):
     """The fused kernel delivers delay-1 records in the cycle and the rest one
