@@ -671,9 +671,12 @@ __device__ inline void far_phase(const NetDev& nd, uint32_t Fp, const uint32_t* 
 }
 
 /* IZO: every neuron of the network is an Izhikevich neuron, so only its 5
- * parameters and 2 state variables are held in registers. */
+ * parameters and 2 state variables are held in registers. Always inlined
+ * into the kernels: as a call, the network descriptor is copied to scratch
+ * (about 500 B per lane), which the compiler chooses by itself once the
+ * loop grows past its inlining threshold. */
 template<bool SAT, bool IZO>
-__device__ void run_partition(const NetDev& nd, uint32_t p)
+__device__ __forceinline__ void run_partition(const NetDev& nd, uint32_t p)
 {
 	constexpr int NP = IZO ? 5 : MAX_PARAM;
 	constexpr int NS = IZO ? 2 : MAX_STATE;
