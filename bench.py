@@ -635,7 +635,7 @@ def main():
     ap.add_argument("--synapses", type=int, default=1000)
     ap.add_argument("--partition-size", type=int, default=0, help="neurons per workgroup (0: automatic)")
     ap.add_argument("--exchange-waves", type=int, default=0,
-                    help="waves per workgroup running the exchange (0: none)")
+                    help="waves per workgroup running the exchange (0: automatic, < 0: none)")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")

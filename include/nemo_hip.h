@@ -70,11 +70,13 @@ nemo_status_t nemo_set_device_construction(nemo_configuration_t conf, int enable
 
 /* Waves of each step-kernel workgroup that run the exchange between
  * partitions and the delivery of delay-1 records, while the others deliver
- * the previous cycle's records of longer delays: n > 0 = n waves (at least
- * one other must remain; needs deferred delivery and LDS room for the far
- * list), 0 (default) or < 0 = none (every wave does both, one after the
- * other). Same results either way. No counterpart in NeMo; a tuning knob of
- * the fused step kernel. */
+ * the previous cycle's records of longer delays (and, with STDP, build the
+ * work lists and accumulate): n > 0 = n waves (at least one other must
+ * remain; needs deferred delivery and LDS room for the far list), < 0 =
+ * none (every wave does both, one after the other), 0 (default) =
+ * automatic: 3 of every 8 waves for networks with STDP and at least 8
+ * waves (512 neurons) per workgroup, none otherwise. Same results either
+ * way. No counterpart in NeMo; a tuning knob of the fused step kernel. */
 nemo_status_t nemo_set_exchange_waves(nemo_configuration_t conf, int waves);
 
 /* Which step implementation runs a simulation:

@@ -156,7 +156,7 @@ runBatch(Simulation* const sims[], size_t n, const unsigned nsteps[], unsigned l
 			Simulation* sim = sims[order[j]];
 			if(sim->device() != dev || sim->variant() != variant) break;
 			const uint32_t bd = std::max(blockDim, sim->blockThreads());
-			const uint32_t ld = std::max(lds, sim->ldsBytes());
+			const uint32_t ld = std::max(lds, sim->batchLdsBytes());
 			uint32_t maxB = 0;
 			hipCheck(nemo_dev_step_max_blocks(bd, ld, variant, &maxB), "occupancy query");
 			if(blocks + sim->partitions() > maxB) {
@@ -177,7 +177,7 @@ runBatch(Simulation* const sims[], size_t n, const unsigned nsteps[], unsigned l
 		for(size_t k : members) {
 			Simulation* sim = sims[k];
 			const uint32_t idx = uint32_t(nets.size());
-			nets.push_back(sim->prepareLaunch(nsteps[k]));
+			nets.push_back(sim->prepareLaunch(nsteps[k], true));
 			for(uint32_t p = 0; p < sim->partitions(); ++p) items.push_back(std::make_pair(idx, p));
 		}
 		const std::vector<WorkItem> work = Simulation::xcdWorkTable(items);

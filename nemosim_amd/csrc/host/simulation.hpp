@@ -108,9 +108,15 @@ class Simulation
 		uint32_t partitions() const { return m_P; }
 		uint32_t blockThreads() const { return m_T; }
 		uint32_t ldsBytes() const { return m_lds; }
+		/* LDS of a batched launch: without the far list of exchange waves
+		 * the library chose by itself (a batched launch runs every
+		 * workgroup at its largest member's LDS, so one network's far list
+		 * would take co-residency from all the others) */
+		uint32_t batchLdsBytes() const { return m_exAuto ? m_ldsNoFar : m_lds; }
 		/* prepare a launch of nsteps cycles: fills and returns the device
-		 * descriptor (host copy) with per-launch fields set */
-		NetDev prepareLaunch(unsigned nsteps);
+		 * descriptor (host copy) with per-launch fields set; batched: as
+		 * one member of a batched launch (see batchLdsBytes) */
+		NetDev prepareLaunch(unsigned nsteps, bool batched = false);
 		/* advance the cycle count by a finished launch; throws on a status
 		 * error. ctl holds the first 8 words of the control block as the
 		 * launch left them, counted from zero at the launch's start (a
@@ -176,6 +182,8 @@ class Simulation
 		uint32_t m_stdpCache = 0;    /* STDP chunks staged in LDS per cycle */
 		bool m_prefRing = false;     /* STDP list prefixes of postLen cycles kept in LDS */
 		uint32_t m_exWaves = 0;      /* exchange waves (NetDev.exWaves) */
+		bool m_exAuto = false;       /* exchange waves chosen by the library, not the configuration */
+		uint32_t m_ldsNoFar = 0;     /* m_lds without the far list */
 		uint32_t m_farCap = 0;       /* far-list capacity in sources (NetDev.farCap) */
 		void planExchange(const ConfigurationImpl& conf, uint32_t ldsLimit);
 		/* every neuron is an Izhikevich neuron (NEMO_NET_IZHIKEVICH_ONLY) */

@@ -102,7 +102,8 @@ class Configuration(object):
 
     def set_exchange_waves(self, waves):
         """Waves per step-kernel workgroup running the exchange and delay-1
-        delivery (0 or < 0 = none, the default; same results)."""
+        delivery (< 0 = none; 0 = automatic, the default: 3 of 8 for STDP
+        networks of at least 8 waves per workgroup; same results)."""
         _check(self._lib.nemo_set_exchange_waves(self._h, int(waves)))
 
     def set_device_construction(self, enable):

@@ -185,6 +185,36 @@ def test_exchange_waves(nemo_mod, oracle_mod, waves, stdp_on):
         compare_weights(sim, orc, spec, 5000, seed=3)
 
 
+def test_automatic_exchange_waves_single_and_batched(nemo_mod, oracle_mod):
+    """An STDP network of 512-neuron workgroups gets exchange waves by
+    default (3 of 8) in its own launches, and none in batched launches,
+    whose LDS it would otherwise raise for every member. Launches of both
+    kinds alternate, with applyStdp between some, against the oracle:
+    rasters, state and weights."""
+    spec = examples.izhikevich_80_20(6000, 300, seed=37, stdp=True)
+    pre, post = examples.stdp_window()
+    stdp = (pre, post, 0.0, 1.0, 0.0, -1.0)
+    sim = hip_sim(nemo_mod, spec, stdp=stdp, partition_size=512)
+    info = sim.info()
+    assert info["exchange_waves"] == 3 and info["far_list_sources"] >= 256, info
+    orc = oracle_mod.from_spec(spec, stdp=stdp)
+    for k, batched, apply in [(120, False, True), (90, True, False), (75, False, False), (150, True, True),
+                              (60, True, False), (100, False, False)]:
+        if batched:
+            nemo_mod.run_batch([sim], [k])
+            gc, gi = sim.read_fired()
+            rc, ri = orc.run(k)
+            d = first_divergence(rc, ri, gc, gi)
+            assert d is None, "first diverging cycle %d: %s" % d
+        else:
+            compare_run(sim, orc, k)
+        if apply:
+            sim.apply_stdp(1.0)
+            orc.apply_stdp(1.0)
+    compare_state(sim, orc, spec.n, 2, sample=range(0, spec.n, 13))
+    compare_weights(sim, orc, spec, 4000, seed=5)
+
+
 @pytest.mark.parametrize("staging", [True, False])
 def test_stdp_work_lists(nemo_mod, oracle_mod, staging):
     """STDP work lists with and without LDS staging (count-prefix ring and
