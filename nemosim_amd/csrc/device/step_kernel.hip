@@ -499,7 +499,7 @@ __device__ inline unsigned long long exchange_phase(const NetDev& nd, uint32_t p
 		uint32_t slot, uint32_t flb, uint32_t wave, uint32_t lane, uint32_t* pref, uint32_t* exFlag,
 		uint32_t* FA, uint32_t* FB, uint32_t* fcnt, uint32_t* pringDst, const NEMO_GLOBAL uint2* syn,
 		const NEMO_GLOBAL uint32_t* off, uint32_t offMask, int32_t* ringE, int32_t* ringI, uint32_t* ovf,
-		uint32_t fwords)
+		uint32_t fwords, const uint32_t* listDone, uint32_t listTarget)
 {
 	constexpr uint32_t EX_K = ExSources<SAT, IZO>::value;
 	const uint32_t P = nd.P, T = nd.T, D = nd.D, exw = nd.exWaves, cap = nd.farCap;
@@ -520,6 +520,11 @@ __device__ inline unsigned long long exchange_phase(const NetDev& nd, uint32_t p
 	const uint32_t el = wave * 64u + lane;
 	if(el == 0) *fcnt = lim;
 	if(pringDst) {
+		/* the slot still holds the prefix the far waves build this cycle's
+		 * STDP list from: wait until they have */
+		while(__hip_atomic_load(listDone, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) < listTarget) {
+			__builtin_amdgcn_s_sleep(1);
+		}
 		for(uint32_t i = el; i <= P; i += nEl) pringDst[i] = pref[i];
 	}
 	const NEMO_GLOBAL uint64_t* fl = glb(nd.fl) + size_t(flb) * P * T;
@@ -823,9 +828,15 @@ __device__ __forceinline__ void run_partition(const NetDev& nd, uint32_t p)
 	uint32_t* farB = farA + 2u * fcap;
 	uint32_t* fcnt = farB + 2u * fcap;      /* [2], by cycle parity */
 	uint32_t* exFlag = fcnt + 2;
+	/* with exchange waves and the early STDP list: far waves that have built
+	 * their share of the cycle's list (records staged), counted over the
+	 * launch (the word after the flag) */
+	uint32_t* listDone = exFlag + 1;
+	const uint32_t nfar = nwaves - exw;
 	if(exw && tid == 0) {
 		fcnt[0] = fcnt[1] = 0u;
 		*exFlag = 0u;
+		*listDone = 0u;
 	}
 	uint32_t rslot = pring ? uint32_t(nd.cycle0 % postLen) : 0u;   /* t mod postLen */
 	if(stdp && tid == 0) {
@@ -1014,32 +1025,36 @@ __device__ __forceinline__ void run_partition(const NetDev& nd, uint32_t p)
 			const uint32_t* sp = pring ? pringArea + rslot * PW : spref;
 			const uint32_t Fc = sp[P];
 			const NEMO_GLOBAL uint64_t* srw = glb(nd.srow) + size_t(cb) * P * T;
-			for(uint32_t j0 = wave * 64u; j0 < Fc; j0 += nthreads) {   /* wave-uniform */
-				stdp_list_build(j0, Fc, sp, P, p, T, srw, glb(nd.rcm_sd), lane, nd.stdpCache,
-						nwork + (t & 1u), work, sdcache);
+			/* with exchange waves the far waves build it, so that the
+			 * exchange starts at once */
+			const uint32_t lw0 = exw ? wave - exw : wave, lnw = exw ? nfar : nwaves;
+			if(!exw || wave >= exw) {
+				for(uint32_t j0 = lw0 * 64u; j0 < Fc; j0 += lnw * 64u) {   /* wave-uniform */
+					stdp_list_build(j0, Fc, sp, P, p, T, srw, glb(nd.rcm_sd), lane, nd.stdpCache,
+							nwork + (t & 1u), work, sdcache);
+				}
 			}
-			/* the staged records land before the exchange barrier: after it,
-			 * any wave may read them (wave 0's gather waits for them anyway) */
-			if(wave != 0) __builtin_amdgcn_s_waitcnt(0);
+			/* the staged records land before the exchange barrier, or,
+			 * with exchange waves, before the wave is counted: after that,
+			 * any wave may read them (wave 0's gather waits for them anyway;
+			 * exchange waves stage none) */
+			if(exw ? wave >= exw : wave != 0) __builtin_amdgcn_s_waitcnt(0);
+		}
+		if(early && exw && wave >= exw && lane == 0) {
+			__hip_atomic_fetch_add(listDone, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
 		}
 
 		if(exw) {
 			/* ---- 3+4 with exchange waves: the exchange and this cycle's
 			 * delay-1 records on waves [0, exw), the far list of cycle t-1
 			 * on the others (see exchange_phase) */
-			if(early && t >= postLen) {
-				/* the work list is complete and its staged records have
-				 * landed before the accumulation reads them */
-				__builtin_amdgcn_s_waitcnt(0);
-				__syncthreads();
-			}
 			const uint32_t cb = uint32_t(t & 1u);
 			if(wave < exw) {
 				/* the prefix is kept for the list of cycle t+postLen (slot t
 				 * mod postLen, whose cycle t-postLen list was built above) */
 				events += exchange_phase<SAT, IZO>(nd, p, t, k, slot, flb, wave, lane, pref, exFlag,
 						farA + cb * fcap, farB + cb * fcap, fcnt + cb, pring ? pringArea + rslot * PW : nullptr,
-						syn, off, offMask, ringE, ringI, ovf, fwords);
+						syn, off, offMask, ringE, ringI, ovf, fwords, listDone, (k + 1u) * nfar);
 			} else {
 				far_phase<SAT, IZO>(nd, fcnt[cb ^ 1u], farA + (cb ^ 1u) * fcap, farB + (cb ^ 1u) * fcap,
 						wave - exw, nwaves - exw, lane, (slot + D - 1u) % D, syn, ringE, ringI, ovf, fwords);
@@ -1155,7 +1170,15 @@ __device__ __forceinline__ void run_partition(const NetDev& nd, uint32_t p)
 		 * delivery is done, taking STDP_K chunks at a time from a shared
 		 * counter (the waves that finish delivery first take more). */
 		if(stdp) {
-			const uint32_t listed = early ? nwork[t & 1u] : 0u;   /* final since the exchange barrier */
+			if(early && exw) {
+				/* the list is complete and its staged records have landed
+				 * once every far wave has counted itself */
+				while(__hip_atomic_load(listDone, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) < (k + 1u) * nfar) {
+					__builtin_amdgcn_s_sleep(1);
+				}
+			}
+			/* final since the exchange barrier (or the count above) */
+			const uint32_t listed = early ? nwork[t & 1u] : 0u;
 			const bool late = t >= postLen && (!early || listed > NEMO_STDP_WORK_CAP);
 			if(late) {
 				/* the list is built here, with the targets' entries and rows
