@@ -215,15 +215,18 @@ def test_automatic_exchange_waves_single_and_batched(nemo_mod, oracle_mod):
     compare_weights(sim, orc, spec, 4000, seed=5)
 
 
+@pytest.mark.parametrize("exchange", [0, 1])
 @pytest.mark.parametrize("staging", [True, False])
-def test_stdp_work_lists(nemo_mod, oracle_mod, staging):
+def test_stdp_work_lists(nemo_mod, oracle_mod, staging, exchange):
     """STDP work lists with and without LDS staging (count-prefix ring and
     reverse records in LDS), in one-cycle launches (each launch rebuilds
     the ring from the counts of the last postLen cycles) and with bursts of
     3000 forced neurons: postLen cycles later each of the 32 partitions has
     about 1200 chunks of 64 synapses to accumulate, more than its list
     holds (1024), so the list is rebuilt after delivery and the excess is
-    accumulated inline. Rasters and weights bit-exact."""
+    accumulated inline. With one exchange wave of the two, the other wave
+    builds the list and counts itself in LDS in place of a barrier. Rasters
+    and weights bit-exact."""
     n = 4000
     spec = examples.izhikevich_80_20(n, 1000, seed=21, stdp=True)
     pre, post = examples.stdp_window()
@@ -232,9 +235,12 @@ def test_stdp_work_lists(nemo_mod, oracle_mod, staging):
     conf.set_stdp_function(pre, post, min_e=0.0, max_e=1.0, min_i=0.0, max_i=-1.0)
     conf.set_partition_size(128)
     conf.set_stdp_lds_staging(staging)
+    if exchange:
+        conf.set_exchange_waves(exchange)
     sim = nemo_mod.Simulation(net, conf)
     info = sim.info()
     assert info["partitions"] == 32
+    assert info["exchange_waves"] == exchange, info
     assert info["stdp_prefix_ring"] == (1 if staging else 0), info
     assert (info["stdp_staged_chunks"] > 0) == staging, info
     orc = oracle_mod.from_spec(spec, stdp=(pre, post, 0.0, 1.0, 0.0, -1.0))
