@@ -647,7 +647,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=200)
     ap.add_argument("--serve-budget-gb", type=float, default=64.0)
     ap.add_argument("--serve-chunk", type=int, default=100)
-    ap.add_argument("--serve-workers", type=int, default=8)
+    ap.add_argument("--serve-workers", type=int, default=16,
+                    help="serving: host threads building requests (the GPU host's 16-core share)")
     ap.add_argument("--serve-rate", type=float, default=50.0, help="serving: Poisson arrival rate (requests/s)")
     ap.add_argument("--serve-device-construction", action="store_true",
                     help="serving: build each network on the device instead of the host")

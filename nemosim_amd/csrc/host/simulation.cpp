@@ -424,10 +424,18 @@ Simulation::Simulation(const NetworkImpl& net, const ConfigurationImpl& conf) :
 		m_words = m_words32 / 2;
 	} else {
 		m_words = m_P * (T / 64u);
-		/* lanes per fired source in delivery: a power of two about 1.5x the
-		 * mean bucket length (synapses of one source into one partition);
-		 * each lane takes two records per pass, so one pass covers almost
-		 * every bucket (measured at 200k x 1000: 8 lanes beat 4) */
+		/* lanes per fired source in delivery, from the mean bucket length
+		 * (synapses of one source into one partition): a power of two about
+		 * 1.5x of it for short buckets, so that one pass (two records per
+		 * lane) covers almost every bucket (200k x 1000, buckets of 4: 8
+		 * lanes beat 4 and 2); for long buckets no more than half of it,
+		 * and at least 8, since every round serves only 64/lanes sources
+		 * and a wave with many fired sources runs as many rounds. Measured
+		 * per cycle, the capped count against 1.5x: 20k x 1000 (buckets of
+		 * 25) 8.6 against 10.9 us; IF_curr_exp_noise at 333 Hz, 3k x 300
+		 * (12.5) 28.1 against 48.8, 12k x 550 (42) 57.8 against 74.1,
+		 * 50k x 1000 (10) 145.5 against 165.8; 3k x 300 at 6 Hz 6.7
+		 * against 6.1. */
 		const double lavg = m_n ? double(m_nsyn) / (double(m_n) * double(m_P)) : 0.0;
 		m_glog2 = 0;
 		if(conf.deliveryLanes) {
@@ -436,6 +444,11 @@ Simulation::Simulation(const NetworkImpl& net, const ConfigurationImpl& conf) :
 			while(m_glog2 < 6 && double(1u << m_glog2) < 1.5 * lavg) {
 				++m_glog2;
 			}
+			uint32_t half = 0;
+			while(half < 6 && double(1u << half) < 0.5 * lavg) {
+				++half;
+			}
+			m_glog2 = std::min(m_glog2, std::max(3u, half));
 		}
 	}
 
