@@ -188,7 +188,8 @@ typedef struct nemo_hip_info {
 	unsigned long long device_bytes;
 	unsigned lds_bytes;
 	unsigned block_threads;
-	float last_kernel_ms;        /* step-kernel time of the last run (HIP events) */
+	float last_kernel_ms;        /* step-kernel time of the last run: HIP events for its own launches;
+	                              * in a batched run, its workgroups' first start to last end */
 	unsigned fused;              /* 1: fused persistent step kernel; 0: plugin step */
 	unsigned deferred;           /* 1: records of delay >= 2 delivered one cycle late */
 	unsigned lanes_per_source;   /* lanes sharing one fired source in delivery */

@@ -32,7 +32,9 @@ extern "C" {
 /* The per-network control block (16 words): [0] workgroups started, [1]
  * status, [2..3] delivered-synapse count, and, for the first timeout,
  * [4] cycle, [5] waiting partition, [6] awaited partition, [7] workgroups
- * started by then. */
+ * started by then; [8..9] the earliest start and [10..11] the latest end
+ * of its workgroups on the device wall clock (read after batched
+ * launches, whose graph initialises them). */
 #define NEMO_CTL_WORDS 16u
 
 /*
@@ -122,8 +124,11 @@ int nemo_dev_step_launch_one(const NetDev* nd_host, const WorkItem* work_dev, ui
 		uint32_t blockDim, uint32_t ldsBytes, uint32_t variant, void* stream);
 int nemo_dev_step_max_blocks(uint32_t blockDim, uint32_t ldsBytes, uint32_t variant, uint32_t* maxBlocks);
 /* Batched step as an instantiated HIP graph (clear -> persistent step ->
- * collect); collect_dev receives [nnets][8] words: the first 8 words of each
- * network's control block. The graph reads nets_dev/work_dev at replay time. */
+ * collect); collect_dev receives [nnets][16] words: the first 12 words of
+ * each network's control block, the last four of them the first start and
+ * the last end of its workgroups on the device wall clock (64-bit each; the
+ * graph initialises them before the step kernel). The graph reads
+ * nets_dev/work_dev at replay time. */
 int nemo_dev_batch_graph_create(const NetDev* nets_dev, uint32_t nnets, const WorkItem* work_dev,
 		uint32_t nblocks, uint32_t blockDim, uint32_t ldsBytes, uint32_t variant, uint32_t* collect_dev,
 		void** exec_out);

@@ -724,9 +724,13 @@ __device__ __forceinline__ void run_partition(const NetDev& nd, uint32_t p)
 	const NEMO_GLOBAL uint2* syn = glb(reinterpret_cast<const uint2*>(nd.syn));
 	const NEMO_GLOBAL uint32_t* off = glb(nd.off);
 
-	/* count the workgroups of the network that have started (diagnostics) */
+	/* count the workgroups of the network that have started (diagnostics),
+	 * and keep the first start on the device wall clock (control block
+	 * words 8..9; with the last end in 10..11, the network's time in a
+	 * batched launch, whose graph initialises both) */
 	if(tid == 0) {
 		atomicAdd(nd.status - 1, 1u);
+		atomicMin(reinterpret_cast<unsigned long long*>(nd.status + 7), (unsigned long long) wall_clock64());
 	}
 
 	/* restore the pending-current ring: 1 KB per wave instruction straight
@@ -1303,6 +1307,10 @@ __device__ __forceinline__ void run_partition(const NetDev& nd, uint32_t p)
 	if(lane == 0 && events) {
 		atomicAdd(nd.events, events);
 	}
+	__syncthreads();
+	if(tid == 0) {
+		atomicMax(reinterpret_cast<unsigned long long*>(nd.status + 9), (unsigned long long) wall_clock64());
+	}
 }
 
 } // namespace
@@ -1383,17 +1391,19 @@ __global__ void nemo_batch_clear_kernel(const NetDev* __restrict__ nets, uint32_
 	for(uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
 		uint32_t* ctl = nets[i].status - 1;
 		for(uint32_t w = 0; w < 8u; ++w) ctl[w] = 0u;
+		ctl[8] = ctl[9] = ~0u;   /* first start: none yet */
+		ctl[10] = ctl[11] = 0u;  /* last end */
 	}
 }
 
-/* Batched launch, last node: gather the first 8 words of each network's
- * control block (status, events, diagnostics) into out[8*i .. 8*i+7], so the
- * host reads all of them with one copy. */
+/* Batched launch, last node: gather the first 12 words of each network's
+ * control block (status, events, diagnostics, first start and last end)
+ * into out[16*i .. 16*i+11], so the host reads all of them with one copy. */
 __global__ void nemo_batch_collect_kernel(const NetDev* __restrict__ nets, uint32_t count, uint32_t* __restrict__ out)
 {
 	for(uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
 		const uint32_t* ctl = nets[i].status - 1;
-		for(uint32_t w = 0; w < 8u; ++w) out[8 * i + w] = ctl[w];
+		for(uint32_t w = 0; w < 12u; ++w) out[16 * i + w] = ctl[w];
 	}
 }
 

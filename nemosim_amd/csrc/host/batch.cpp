@@ -40,7 +40,7 @@ struct BatchPlan {
 	std::vector<WorkItem> work;
 	DevBuf dnets, dwork, dcollect;
 	NetDev* hnets = nullptr;       /* pinned */
-	uint32_t* hcollect = nullptr;  /* pinned, [nnets][8] */
+	uint32_t* hcollect = nullptr;  /* pinned, [nnets][16] */
 	void* exec = nullptr;
 
 	bool matches(int dev, uint32_t var, uint32_t bd, uint32_t ld, uint32_t nn, const std::vector<WorkItem>& w) const
@@ -78,6 +78,19 @@ batchStream(int dev)
 	return g_streams[dev];
 }
 
+/* the device wall clock's rate (wall_clock64 in the step kernel), kHz */
+double
+wallClockKHz(int dev)
+{
+	static double g_khz[64] = {};
+	if(g_khz[dev] == 0.0) {
+		int khz = 0;
+		hipCheck(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev), "hipDeviceGetAttribute");
+		g_khz[dev] = khz > 0 ? double(khz) : 100000.0;
+	}
+	return g_khz[dev];
+}
+
 BatchPlan&
 getPlan(int dev, uint32_t variant, uint32_t blockDim, uint32_t lds, uint32_t nnets,
 		const std::vector<WorkItem>& work)
@@ -97,9 +110,9 @@ getPlan(int dev, uint32_t variant, uint32_t blockDim, uint32_t lds, uint32_t nne
 	plan->work = work;
 	plan->dnets.alloc(size_t(nnets) * sizeof(NetDev));
 	plan->dwork.upload(work.data(), work.size() * sizeof(WorkItem));
-	plan->dcollect.alloc(size_t(nnets) * 32);
+	plan->dcollect.alloc(size_t(nnets) * 64);
 	hipCheck(hipHostMalloc(reinterpret_cast<void**>(&plan->hnets), size_t(nnets) * sizeof(NetDev)), "hipHostMalloc");
-	hipCheck(hipHostMalloc(reinterpret_cast<void**>(&plan->hcollect), size_t(nnets) * 32), "hipHostMalloc");
+	hipCheck(hipHostMalloc(reinterpret_cast<void**>(&plan->hcollect), size_t(nnets) * 64), "hipHostMalloc");
 	hipCheck(nemo_dev_batch_graph_create(plan->dnets.as<NetDev>(), nnets, plan->dwork.as<WorkItem>(),
 				uint32_t(work.size()), blockDim, lds, variant, plan->dcollect.as<uint32_t>(), &plan->exec),
 			"batched step graph");
@@ -186,7 +199,7 @@ runBatch(Simulation* const sims[], size_t n, const unsigned nsteps[], unsigned l
 		hipCheck(hipMemcpyAsync(plan.dnets.ptr, plan.hnets, nets.size() * sizeof(NetDev),
 					hipMemcpyHostToDevice, st), "H2D");
 		hipCheck(nemo_dev_graph_launch(plan.exec, st), "batched step graph launch");
-		hipCheck(hipMemcpyAsync(plan.hcollect, plan.dcollect.ptr, nets.size() * 32,
+		hipCheck(hipMemcpyAsync(plan.hcollect, plan.dcollect.ptr, nets.size() * 64,
 					hipMemcpyDeviceToHost, st), "D2H");
 		hipCheck(hipStreamSynchronize(st), "hipStreamSynchronize");
 		/* every member ran: advance all of them, then report the first error */
@@ -194,10 +207,14 @@ runBatch(Simulation* const sims[], size_t n, const unsigned nsteps[], unsigned l
 		exception firstError(NEMO_OK, "");
 		for(size_t m = 0; m < members.size(); ++m) {
 			const size_t k = members[m];
-			const uint32_t* c = plan.hcollect + 8 * m;
+			const uint32_t* c = plan.hcollect + 16 * m;
 			if(events) {
 				events[k] = (unsigned long long) c[2] | ((unsigned long long) c[3] << 32);
 			}
+			/* its workgroups' first start to last end on the device clock */
+			const uint64_t t0 = (uint64_t) c[8] | ((uint64_t) c[9] << 32);
+			const uint64_t t1 = (uint64_t) c[10] | ((uint64_t) c[11] << 32);
+			sims[k]->setKernelMs(t1 > t0 ? float(double(t1 - t0) / wallClockKHz(dev)) : 0.0f);
 			try {
 				/* the graph cleared the block before the launch: c counts
 				 * from zero, and the block keeps these values afterwards */

@@ -113,6 +113,9 @@ class Simulation
 		 * workgroup at its largest member's LDS, so one network's far list
 		 * would take co-residency from all the others) */
 		uint32_t batchLdsBytes() const { return m_exAuto ? m_ldsNoFar : m_lds; }
+		/* the step-kernel time of the last run, for a batched one: from the
+		 * first start to the last end of this network's workgroups */
+		void setKernelMs(float ms) { m_kernelMs = ms; }
 		/* prepare a launch of nsteps cycles: fills and returns the device
 		 * descriptor (host copy) with per-launch fields set; batched: as
 		 * one member of a batched launch (see batchLdsBytes) */

@@ -323,6 +323,14 @@ class Simulation(object):
         _check(self._lib.nemo_hip_get_info(self._h, ctypes.byref(i)))
         return {f: getattr(i, f) for f, _ in _lib.InfoT._fields_}
 
+    def last_kernel_ms(self):
+        """Step-kernel time of the last run: HIP events for the simulation's
+        own launches; in a batched run, from the first start to the last end
+        of its workgroups on the device clock."""
+        i = _lib.InfoT()
+        _check(self._lib.nemo_hip_get_info(self._h, ctypes.byref(i)))
+        return float(i.last_kernel_ms)
+
     def get_membrane_potential(self, idx):
         v = ctypes.c_float()
         _check(self._lib.nemo_get_membrane_potential(self._h, int(idx), ctypes.byref(v)))

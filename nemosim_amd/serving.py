@@ -52,6 +52,7 @@ class Request(object):
         self.counts = []
         self.ids = []
         self.sim = None
+        self.cost_ms = None   # step-kernel ms per cycle in its last batched launch
 
     @property
     def latency_s(self):
@@ -132,11 +133,13 @@ def poisson_requests(count, rate_hz, seed, size_range=(1000, 50000), syn_range=(
 
 class Server(object):
     def __init__(self, hbm_budget_bytes, chunk_cycles=100, build_workers=4, keep_rasters=True,
-                 device_construction=False, tuning=None, keep_finished=False):
+                 device_construction=False, tuning=None, keep_finished=False, balance=True):
         """keep_finished: a finished request keeps its simulation (built
         with synapse queries on) for inspection after serve(); its budget
-        is released all the same."""
+        is released all the same. balance: cycles per chunk by each
+        request's measured cost (see _advance)."""
         self.keep_finished = keep_finished
+        self.balance = balance
         self.budget = int(hbm_budget_bytes)
         self.device_construction = device_construction
         self.tuning = tuning
@@ -160,10 +163,21 @@ class Server(object):
 
     def _advance(self, active):
         """One chunk for every resident simulation; STDP requests stop at
-        their applyStdp boundaries."""
+        their applyStdp boundaries. A batched launch lasts as long as its
+        slowest network, so with balance each request runs about as many
+        cycles as fit in the time the median request needs for chunk
+        cycles, by the step-kernel time per cycle each one took in its last
+        chunk (between 1 and 4 chunks' worth; chunk cycles until measured):
+        networks that fire fast or are large no longer hold the others'
+        CUs idle. Results do not depend on how the cycles are cut."""
+        known = [r.cost_ms for r in active if r.cost_ms]
+        budget = self.chunk * float(np.median(known)) if (self.balance and known) else None
         sims, steps = [], []
         for r in active:
-            k = min(self.chunk, r.duration_ms - r.cycles_done)
+            k = self.chunk
+            if budget is not None and r.cost_ms:
+                k = int(max(1, min(4 * self.chunk, round(budget / r.cost_ms))))
+            k = min(k, r.duration_ms - r.cycles_done)
             if r.stdp:
                 k = min(k, r.stdp_period - (r.cycles_done % r.stdp_period))
             sims.append(r.sim)
@@ -175,6 +189,10 @@ class Server(object):
         due = []
         for r, k, e in zip(active, steps, ev):
             r.events += int(e)
+            if self.balance and k > 0:
+                ms = r.sim.last_kernel_ms()
+                if ms > 0.0:
+                    r.cost_ms = ms / k
             if self.keep_rasters:
                 c, ids = r.sim.read_fired()
                 r.counts.append(c.copy())
