@@ -446,7 +446,7 @@ def bench_serve(args):
     from nemosim_amd import autotune
     srv = serving.Server(hbm_budget_bytes=budget, chunk_cycles=args.serve_chunk,
                          build_workers=args.serve_workers, keep_rasters=False,
-                         device_construction=args.serve_device_construction,
+                         device_construction=not args.serve_host_construction,
                          tuning=autotune.load_table() if args.serve_tuned else None,
                          balance=not args.serve_no_balance)
     log("serve: %d requests, budget %.0f GiB, chunks of %d cycles" % (len(reqs), args.serve_budget_gb, args.serve_chunk))
@@ -495,7 +495,7 @@ def bench_serve(args):
                 "30%% IF_curr_exp_noise)" % args.serve_rate,
         "config": {"workload": "serving", "requests": stats["requests"], "budget_gib": args.serve_budget_gb,
                    "chunk_cycles": args.serve_chunk, "build_workers": args.serve_workers,
-                   "device_construction": bool(args.serve_device_construction),
+                   "device_construction": not args.serve_host_construction,
                    "balanced_chunks": not args.serve_no_balance},
         "p50_latency_s": stats["p50_latency_s"],
         "p99_latency_s": stats["p99_latency_s"],
@@ -652,8 +652,8 @@ def main():
     ap.add_argument("--serve-workers", type=int, default=16,
                     help="serving: host threads building requests (the GPU host's 16-core share)")
     ap.add_argument("--serve-rate", type=float, default=50.0, help="serving: Poisson arrival rate (requests/s)")
-    ap.add_argument("--serve-device-construction", action="store_true",
-                    help="serving: build each network on the device instead of the host")
+    ap.add_argument("--serve-host-construction", action="store_true",
+                    help="serving: build each network's synapse structures on the host instead of the device")
     ap.add_argument("--serve-no-balance", action="store_true",
                     help="serving: the same chunk of cycles for every request (no balancing by measured cost)")
     ap.add_argument("--serve-tuned", action="store_true", help="serving: partition sizes and delivery lanes "
