@@ -167,7 +167,8 @@ class Server(object):
         slowest network, so with balance each request runs about as many
         cycles as fit in the time the median request needs for chunk
         cycles, by the step-kernel time per cycle each one took in its last
-        chunk (between 1 and 4 chunks' worth; chunk cycles until measured):
+        chunk (between a tenth of a chunk and 4 chunks' worth, so that no
+        request is starved; chunk cycles until measured):
         networks that fire fast or are large no longer hold the others'
         CUs idle. Results do not depend on how the cycles are cut."""
         known = [r.cost_ms for r in active if r.cost_ms]
@@ -176,7 +177,7 @@ class Server(object):
         for r in active:
             k = self.chunk
             if budget is not None and r.cost_ms:
-                k = int(max(1, min(4 * self.chunk, round(budget / r.cost_ms))))
+                k = int(max(max(1, self.chunk // 10), min(4 * self.chunk, round(budget / r.cost_ms))))
             k = min(k, r.duration_ms - r.cycles_done)
             if r.stdp:
                 k = min(k, r.stdp_period - (r.cycles_done % r.stdp_period))
