@@ -3,6 +3,8 @@ cycle (first diverging cycle reported), neuron state bit-exact (the FP
 arithmetic keeps NeMo's op order; non-finite values compare by class), and
 synapse weights bit-exact under STDP."""
 
+import time
+
 import numpy as np
 import pytest
 
@@ -428,6 +430,27 @@ def test_batch_equals_alone(nemo_mod, oracle_mod):
         gc, gi = sim.read_fired()
         d = first_divergence(rc, ri, gc, gi)
         assert d is None, "network %d: first diverging cycle %d: %s" % ((s.n,) + d)
+
+
+def test_batched_kernel_time_per_network(nemo_mod):
+    """After a batched run each simulation reports its own step-kernel
+    time (its workgroups' first start to last end on the device clock):
+    of two like networks in one launch, the one that runs four times the
+    cycles takes between 2.5 and 6 times as long, and neither takes longer
+    than the launch."""
+    specs = [examples.izhikevich_80_20(4000, 100, seed=500 + i) for i in range(2)]
+    sims = [hip_sim(nemo_mod, s) for s in specs]
+    nemo_mod.run_batch(sims, [1, 1])   # the launch graph is built outside the timed runs
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        nemo_mod.run_batch(sims, [100, 400])
+        wall_ms = (time.perf_counter() - t0) * 1e3
+        a, b = sims[0].last_kernel_ms(), sims[1].last_kernel_ms()
+        assert 0.0 < a < b <= wall_ms, (a, b, wall_ms)
+        times.append(b / a)
+    ratio = sorted(times)[1]
+    assert 2.5 < ratio < 6.0, times
 
 
 def test_apply_stdp_batch_equals_apply_stdp(nemo_mod, oracle_mod):
