@@ -161,19 +161,19 @@ class Server(object):
         """Device memory a request holds only while it is built."""
         return r.construction_bytes() if self.device_construction else 0
 
-    def _advance(self, active):
-        """One chunk for every resident simulation; STDP requests stop at
-        their applyStdp boundaries. A batched launch lasts as long as its
-        slowest network, so with balance each request runs about as many
-        cycles as fit in the time the median request needs for chunk
-        cycles, by the step-kernel time per cycle each one took in its last
-        chunk (between a tenth of a chunk and 4 chunks' worth, so that no
-        request is starved; chunk cycles until measured):
-        networks that fire fast or are large no longer hold the others'
-        CUs idle. Results do not depend on how the cycles are cut."""
+    def plan_steps(self, active):
+        """Cycles each resident request runs in the next chunk. STDP
+        requests stop at their applyStdp boundaries. A batched launch lasts
+        as long as its slowest network, so with balance each request runs
+        about as many cycles as fit in the time the median request needs
+        for chunk cycles, by the step-kernel time per cycle each one took
+        in its last chunk (between a tenth of a chunk and 4 chunks' worth,
+        so that no request is starved; chunk cycles until measured):
+        networks that fire fast or are large no longer hold the others' CUs
+        idle. Results do not depend on how the cycles are cut."""
         known = [r.cost_ms for r in active if r.cost_ms]
         budget = self.chunk * float(np.median(known)) if (self.balance and known) else None
-        sims, steps = [], []
+        steps = []
         for r in active:
             k = self.chunk
             if budget is not None and r.cost_ms:
@@ -181,8 +181,13 @@ class Server(object):
             k = min(k, r.duration_ms - r.cycles_done)
             if r.stdp:
                 k = min(k, r.stdp_period - (r.cycles_done % r.stdp_period))
-            sims.append(r.sim)
             steps.append(k)
+        return steps
+
+    def _advance(self, active):
+        """One chunk for every resident simulation (plan_steps)."""
+        steps = self.plan_steps(active)
+        sims = [r.sim for r in active]
         t0 = time.perf_counter()
         ev = nemo_api.run_batch(sims, steps)
         self.gpu_busy_s += time.perf_counter() - t0

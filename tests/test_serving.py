@@ -23,6 +23,25 @@ def test_budget_admission_order_cpu():
         srv.serve([too_big], realtime=False)
 
 
+def test_balanced_chunk_plan_cpu():
+    """Cycles per chunk by measured cost (host logic, no GPU): unmeasured
+    requests run a whole chunk; measured ones run about chunk x median
+    cost / own cost, clamped to [chunk/10, 4 chunk], never past their
+    duration or an STDP request's applyStdp boundary; without balance every
+    request runs a chunk."""
+    def req(rid, cost, done=0, dur=10000, stdp=False):
+        r = serving.Request(rid, 0.0, 1000, 100, dur, stdp=stdp)
+        r.cost_ms, r.cycles_done = cost, done
+        return r
+    srv = serving.Server(hbm_budget_bytes=1 << 30, chunk_cycles=100)
+    active = [req(0, None), req(1, 0.01), req(2, 0.02), req(3, 0.04), req(4, 0.5), req(5, 0.0001),
+              req(6, 0.01, done=9950), req(7, 0.01, done=1930, stdp=True)]
+    # median of the measured costs: 0.01 ms per cycle, so a budget of 1 ms
+    assert srv.plan_steps(active) == [100, 100, 50, 25, 10, 400, 50, 70]
+    srv.balance = False
+    assert srv.plan_steps(active) == [100, 100, 100, 100, 100, 100, 50, 70]
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("device_construction,tuned", [(False, False), (True, False), (False, True)])
 def test_alone_equals_mix(oracle_mod, device_construction, tuned):
