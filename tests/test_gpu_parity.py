@@ -432,6 +432,34 @@ def test_batch_equals_alone(nemo_mod, oracle_mod):
         assert d is None, "network %d: first diverging cycle %d: %s" % ((s.n,) + d)
 
 
+@pytest.mark.parametrize("stdp_on", [False, True])
+def test_longest_delays_fused(nemo_mod, oracle_mod, stdp_on):
+    """The fused kernel at the longest delays NeMo's CPU backend takes:
+    64 (src/nemo/cpu/Simulation.cpp:52-56), and 44 with STDP, whose
+    postfire window of 20 must fit beside the delay in the 64-cycle firing
+    history. 8000 neurons x 200 synapses, delays uniform up to the limit,
+    700 cycles in uneven launches (with applyStdp between them under STDP):
+    rasters, state and weights bit-exact."""
+    dmax = 44 if stdp_on else 64
+    spec = examples.izhikevich_80_20(8000, 200, seed=41, dmax=dmax, stdp=stdp_on)
+    stdp = None
+    if stdp_on:
+        pre, post = examples.stdp_window()
+        stdp = (pre, post, 0.0, 1.0, 0.0, -1.0)
+    sim = hip_sim(nemo_mod, spec, stdp=stdp)
+    info = sim.info()
+    assert info["fused"] == 1 and info["max_delay"] == dmax, info
+    orc = oracle_mod.from_spec(spec, stdp=stdp)
+    for k in [150, 1, 64, 285, 200]:
+        compare_run(sim, orc, k)
+        if stdp_on:
+            sim.apply_stdp(1.0)
+            orc.apply_stdp(1.0)
+    compare_state(sim, orc, spec.n, 2, sample=range(0, spec.n, 7))
+    if stdp_on:
+        compare_weights(sim, orc, spec, 4000, seed=9)
+
+
 def test_batched_kernel_time_per_network(nemo_mod):
     """After a batched run each simulation reports its own step-kernel
     time (its workgroups' first start to last end on the device clock):
